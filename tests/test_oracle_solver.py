@@ -1,0 +1,301 @@
+"""Pins the oracle's initsolver pieces, Poisson solve and field kernels with
+checks that do not retype the formulas: dense 1-D/3-D operators assembled
+independently (Kronecker sums), exact discrete solutions (linear profiles,
+discrete eigenfunctions), and cross-checks between independent routines
+(fillps -> solve -> correc -> chkdiv; pres_tw_src special cases vs correc)."""
+import numpy as np
+import pytest
+
+from cases import (dense_operator, gauge_free, halo_field, interior, relmax,
+                   second_diff_1d, tri_dense, z_metrics)
+
+
+@pytest.mark.parametrize("bc", ["PP", "NN", "DD", "ND", "DN"])
+@pytest.mark.parametrize("n", [6, 8, 13, 16])
+def test_eigenvalues_are_spectrum_of_1d_operator(oracle, bc, n):
+    """src/initsolver.f90:122-186 ('c'): the eigenvalues are the spectrum of
+    the cell-centred second difference with the BC ghost rules."""
+    lam = oracle.eigenvalues(n, bc, "c")
+    ev = np.linalg.eigvals(second_diff_1d(n, bc, 1.0)).real
+    np.testing.assert_allclose(np.sort(lam), np.sort(ev), rtol=0, atol=1e-12)
+
+
+def test_tridmatrix_uniform_closed_form(oracle):
+    nz, lz = 8, 1.0
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, 0.0, lz)
+    h2 = (nz / lz) ** 2
+    for bc, b0 in (("NN", -h2), ("DD", -3 * h2), ("PP", -2 * h2)):
+        a, b, c = oracle.tridmatrix(bc, nz, 1, dzci, dzfi)
+        np.testing.assert_allclose(a, h2, rtol=1e-14)
+        np.testing.assert_allclose(c, h2, rtol=1e-14)
+        np.testing.assert_allclose(b[1:-1], -2 * h2, rtol=1e-14)
+        np.testing.assert_allclose([b[0], b[-1]], [b0, b0], rtol=1e-14)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.5, 2.0])
+def test_tridmatrix_stretched_invariants(oracle, alpha):
+    """NN rows annihilate constants; dzf-weighted symmetry a/c."""
+    nz = 24
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, alpha, 1.0)
+    a, b, c = oracle.tridmatrix("NN", nz, 1, dzci, dzfi)
+    m = tri_dense(a, b, c, False)
+    assert np.max(np.abs(m @ np.ones(nz))) < 1e-10 * np.max(np.abs(b))
+    wf = dzf[1:nz + 1]
+    np.testing.assert_allclose(wf[:-1] * c[:-1], wf[1:] * a[1:], rtol=1e-13)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+def test_initgrid_invariants(oracle, alpha):
+    n, lz = 32, 2.0
+    dzc, dzf, zc, zf = oracle.initgrid(n, alpha, lz, 1)
+    # arrays cover (0 : n+1)
+    assert abs(np.sum(dzf[1:n + 1]) - lz) < 1e-13
+    assert abs(zf[n] - lz) < 1e-13 and zf[0] == 0.0
+    np.testing.assert_allclose(dzf[1:n + 1], dzf[1:n + 1][::-1], rtol=1e-12)
+    np.testing.assert_allclose(zc[1:n + 1], 0.5 * (zf[0:n] + zf[1:n + 1]), rtol=0, atol=1e-14)
+    # interior centres only: the ghost extension of zc follows the reference
+    # literally (src/initgrid.f90:87-94 sets zc(0) = zf(1) - dzc(1))
+    np.testing.assert_allclose(dzc[1:n], zc[2:n + 1] - zc[1:n], rtol=1e-12)
+    assert dzf[0] == dzf[1]
+    # the top ghost follows src/initgrid.f90:80-83 literally: dzf(n+1) =
+    # dzf(2n-k-1) = dzf(n-2) (unused on the projection path, which reads
+    # dzfi(k), dzci(k) for k <= n only)
+    assert dzf[n + 1] == dzf[n - 2] and dzc[n + 1] == dzc[n - 1]
+    if alpha == 0.0:
+        np.testing.assert_allclose(dzf, lz / n, rtol=1e-13)
+    else:
+        assert dzf[1] < dzf[n // 2]  # clustered at both walls
+
+
+SINGULAR = {"PPPPNN", "NNNNNN", "PPPPPP", "NNPPNN"}
+PATTERNS = ["PPPPNN", "NNNNNN", "PPPPDD", "PPPPPP", "NNPPNN", "DDNNPP",
+            "PPDDNN", "NDPPDD", "DNNNDD", "NNDDDN"]
+
+
+def _build(oracle, ng, cbc, alpha, lengths=(1.0, 1.0, 1.0)):
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, zc, zf = z_metrics(oracle, nz, alpha, lengths[2])
+    dli = (nx / lengths[0], ny / lengths[1], nz / lengths[2])
+    s = oracle.Solver(ng, dli, dzci, dzfi, cbc)
+    return s, dzf, zc
+
+
+@pytest.mark.parametrize("cbc", PATTERNS)
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+def test_solve_matches_dense_solve(oracle, cbc, alpha):
+    ng = (6, 8, 10)
+    s, dzf, _ = _build(oracle, ng, cbc, alpha if cbc[4] != "P" else 0.0,
+                       (1.0, 1.3, 0.9))
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal(ng)
+    if cbc in SINGULAR:
+        w = np.broadcast_to(dzf[1:ng[2] + 1][None, None, :], ng)
+        f = f - np.sum(w * f) / np.sum(w)  # compatible (left null vector)
+    L = dense_operator(ng, cbc, s.dli, s.a, s.b, s.c)
+    fv = f.reshape(-1, order="F")
+    ref = np.linalg.lstsq(L, fv, rcond=None)[0].reshape(ng, order="F")
+    p = halo_field(ng, 1)
+    p[1:-1, 1:-1, 1:-1] = f
+    s.solve(p, pin_gauge=True)
+    pin = interior(p).copy()
+    res = L @ pin.reshape(-1, order="F") - fv
+    assert np.max(np.abs(res)) < 1e-11 * np.max(np.abs(fv))
+    if cbc in SINGULAR:
+        assert relmax(gauge_free(pin), gauge_free(ref)) < 1e-10
+    else:
+        assert relmax(pin, ref) < 1e-11
+
+
+def test_literal_and_pinned_gauge_agree_mod_constant(oracle):
+    ng = (8, 8, 16)
+    s, dzf, _ = _build(oracle, ng, "PPPPNN", 1.5)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(ng)
+    w = np.broadcast_to(dzf[1:17][None, None, :], ng)
+    f -= np.sum(w * f) / np.sum(w)
+    p1, p2 = halo_field(ng, 1), halo_field(ng, 1)
+    p1[1:-1, 1:-1, 1:-1] = f
+    p2[1:-1, 1:-1, 1:-1] = f
+    s.solve(p1, pin_gauge=True)
+    s.solve(p2, pin_gauge=False)
+    assert relmax(gauge_free(interior(p1)), gauge_free(interior(p2))) < 1e-9
+
+
+def test_discrete_eigenfunction_is_reproduced(oracle):
+    """PP/PP/NN uniform: a product of discrete eigenvectors is an exact
+    solution (checks transform pairing, ordering and normfft)."""
+    ng = (16, 12, 10)
+    nx, ny, nz = ng
+    s, _, _ = _build(oracle, ng, "PPPPNN", 0.0, (2.0, 1.0, 1.0))
+    i = np.arange(nx)[:, None, None]
+    j = np.arange(ny)[None, :, None]
+    k = np.arange(nz)[None, None, :]
+    mx, my, mz = 3, 2, 4
+    pe = (np.cos(2 * np.pi * mx * i / nx + 0.3) * np.sin(2 * np.pi * my * j / ny + 0.1) *
+          np.cos(np.pi * mz * (k + 0.5) / nz))
+    # eigenvalues of the dense 1-D operators (not the closed form)
+    lx = np.linalg.eigvalsh(second_diff_1d(nx, "PP", s.dli[0]))
+    ly = np.linalg.eigvalsh(second_diff_1d(ny, "PP", s.dli[1]))
+    lzm = tri_dense(s.a, s.b, s.c, False)
+    vz = np.cos(np.pi * mz * (np.arange(nz) + 0.5) / nz)
+    lz = (lzm @ vz)[0] / vz[0]
+    lamx = -4 * np.sin(np.pi * mx / nx) ** 2 * s.dli[0] ** 2
+    lamy = -4 * np.sin(np.pi * my / ny) ** 2 * s.dli[1] ** 2
+    assert np.min(np.abs(lx - lamx)) < 1e-10 and np.min(np.abs(ly - lamy)) < 1e-10
+    f = (lamx + lamy + lz) * pe
+    p = halo_field(ng, 1)
+    p[1:-1, 1:-1, 1:-1] = f
+    s.solve(p)
+    assert relmax(interior(p), pe) < 1e-12
+
+
+@pytest.mark.parametrize("zbc,vals", [("DD", (0.7, -1.3)), ("NN", (0.4, 0.4))])
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+def test_nonhomogeneous_bc_gives_linear_profile(oracle, zbc, vals, alpha):
+    """bc_rhs + updt_rhs_b + solve (src/initsolver.f90:248-298,
+    src/bound.f90:829-944): zero source with Dirichlet face values v0, v1
+    (or Neumann slope v) has the exact discrete solution linear in z."""
+    ng = (4, 6, 16)
+    lz = 1.0
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, zc, zf = z_metrics(oracle, nz, alpha, lz)
+    dli = (nx, ny, nz)
+    bc = (0.0, 0.0, 0.0, 0.0, vals[0], vals[1])
+    s = oracle.Solver(ng, dli, dzci, dzfi, "PPNN" + zbc, bc=bc)
+    p = halo_field(ng, 1)
+    s.updt_rhs_b(p)
+    s.solve(p)
+    z = zc[1:nz + 1]
+    if zbc == "DD":
+        exact = vals[0] + (vals[1] - vals[0]) * z / lz
+        got = interior(p)
+        assert relmax(got, np.broadcast_to(exact, ng)) < 1e-11
+    else:
+        exact = vals[0] * z
+        got = interior(p)
+        assert relmax(gauge_free(got), gauge_free(np.broadcast_to(exact, ng))) < 1e-10
+    # boundp reproduces the face conditions on the solved field
+    oracle.boundp("PPNN" + zbc, ng, bc, 1, (1 / nx, 1 / ny, 1 / nz), dzc, dzf, p)
+    if zbc == "DD":
+        np.testing.assert_allclose(0.5 * (p[1:-1, 1:-1, 0] + p[1:-1, 1:-1, 1]), vals[0], atol=1e-11)
+        np.testing.assert_allclose(0.5 * (p[1:-1, 1:-1, -1] + p[1:-1, 1:-1, -2]), vals[1], atol=1e-11)
+    else:
+        np.testing.assert_allclose((p[1:-1, 1:-1, 1] - p[1:-1, 1:-1, 0]) / dzc[0], vals[0], atol=1e-9)
+        np.testing.assert_allclose((p[1:-1, 1:-1, -1] - p[1:-1, 1:-1, -2]) / dzc[nz], vals[1], atol=1e-9)
+
+
+def test_boundp_face_rules(oracle):
+    ng = (5, 6, 7)
+    rng = np.random.default_rng(2)
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, 7, 1.2, 1.0)
+    p = halo_field(ng, 1, rng)
+    bc = (0.0, 0.0, 0.25, -0.5, 1.5, 2.0)
+    dl = (0.2, 0.1, 1 / 7)
+    oracle.boundp("PPNDDN", ng, bc, 1, dl, dzc, dzf, p)
+    I = slice(1, -1)
+    np.testing.assert_array_equal(p[0, I, I], p[5, I, I])
+    np.testing.assert_array_equal(p[6, I, I], p[1, I, I])
+    np.testing.assert_allclose((p[I, 1, I] - p[I, 0, I]) / dl[1], 0.25, atol=1e-12)
+    np.testing.assert_allclose(0.5 * (p[I, -1, I] + p[I, -2, I]), -0.5, atol=1e-12)
+    np.testing.assert_allclose(0.5 * (p[I, I, 0] + p[I, I, 1]), 1.5, atol=1e-12)
+    np.testing.assert_allclose((p[I, I, -1] - p[I, I, -2]) / dzc[7], 2.0, atol=1e-11)
+
+
+def _periodic_uvw_halo(u, v, w):
+    """bounduvw for PP x, PP y, no-penetration walls in z (w = 0 on the
+    wall faces k = 0 and k = nz), nh_u = 1."""
+    for a in (u, v, w):
+        a[0, :, :] = a[-2, :, :]
+        a[-1, :, :] = a[1, :, :]
+        a[:, 0, :] = a[:, -2, :]
+        a[:, -1, :] = a[:, 1, :]
+    w[:, :, 0] = 0.0
+    w[:, :, -2] = 0.0
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+def test_projection_step_is_divergence_free(oracle, alpha):
+    """fillps -> solve -> boundp -> correc -> chkdiv (the reference step,
+    main__two_phase_inc_isot.f90:674-695, :773)."""
+    ng = (16, 12, 20)
+    nx, ny, nz = ng
+    lx, ly, lz = 2 * np.pi, np.pi, 1.0
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, alpha, lz)
+    dli = (nx / lx, ny / ly, nz / lz)
+    s = oracle.Solver(ng, dli, dzci, dzfi, "PPPPNN")
+    rng = np.random.default_rng(7)
+    u, v, w = (halo_field(ng, 1, rng) for _ in range(3))
+    _periodic_uvw_halo(u, v, w)
+    dt, rho0 = 0.37, 2.5
+    p = halo_field(ng, 1)
+    oracle.fillps(ng, 1, 1, dli, dzfi, 1 / dt, rho0, u, v, w, p)
+    _, divmax0 = oracle.chkdiv(ng, dli, 1, 1, dzfi, u, v, w)
+    assert divmax0 > 1.0
+    s.updt_rhs_b(p)
+    s.solve(p)
+    oracle.boundp("PPPPNN", ng, (0,) * 6, 1, (1 / dli[0], 1 / dli[1], 1 / dli[2]), dzc, dzf, p)
+    oracle.correc(ng, 1, 1, dli, dzci, dt, rho0, p, u, v, w)
+    _periodic_uvw_halo(u, v, w)
+    divtot, divmax = oracle.chkdiv(ng, dli, 1, 1, dzfi, u, v, w)
+    assert divmax < 1e-11 * divmax0
+    assert abs(divtot) < 1e-9
+
+
+def test_chkdiv_discrete_streamfunction(oracle):
+    ng = (8, 10, 6)
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, 0.0, 1.0)
+    dxi, dyi = 3.0, 5.0
+    rng = np.random.default_rng(9)
+    psi = rng.standard_normal((nx + 3, ny + 3, nz + 2))
+    u = halo_field(ng, 1)
+    v = halo_field(ng, 1)
+    w = halo_field(ng, 1)
+    # edge streamfunction Psi(i,j) = psi[i+1, j+1]:
+    # u(i,j) = (Psi(i,j) - Psi(i,j-1)) dyi, v(i,j) = -(Psi(i,j) - Psi(i-1,j)) dxi
+    u[:, :, :] = (psi[1:, 1:, :] - psi[1:, :-1, :]) * dyi
+    v[:, :, :] = -(psi[1:, 1:, :] - psi[:-1, 1:, :]) * dxi
+    tot, mx = oracle.chkdiv(ng, (dxi, dyi, nz), 1, 1, dzfi, u, v, w)
+    assert mx < 1e-12 and abs(tot) < 1e-11
+    # a uniform expansion u = alpha * x_face has divergence alpha
+    xf = np.arange(-1, nx + 1) / dxi
+    u[:, :, :] = 0.75 * xf[:, None, None]
+    v[:] = 0
+    tot, mx = oracle.chkdiv(ng, (dxi, dyi, nz), 1, 1, dzfi, u, v, w)
+    assert abs(mx - 0.75) < 1e-12 and abs(tot - 0.75 * nx * ny * nz) < 1e-9
+
+
+def test_pres_tw_src_reduces_to_correc(oracle):
+    """src/source.f90:247-309: with rho == rho0 the splitting term is the
+    constant-coefficient gradient; with pold == p and dt == dto it is the
+    variable-coefficient gradient -dt grad(p)/rho_face (src/correc.f90)."""
+    ng = (6, 7, 8)
+    rng = np.random.default_rng(4)
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, 8, 1.5, 1.0)
+    dli = (6.0, 7.0, 8.0)
+    p, pold = halo_field(ng, 1, rng), halo_field(ng, 1, rng)
+    rho = np.asfortranarray(1.0 + 999.0 * rng.uniform(0, 1, size=p.shape))
+    u0, v0, w0 = (halo_field(ng, 1, rng) for _ in range(3))
+    rho0 = 1.0
+    # (a) uniform density
+    rho_u = np.full_like(rho, rho0, order="F")
+    ua, va, wa = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    ub, vb, wb = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    oracle.pres_tw_src(ng, dli, 1, 1, dzci, 1 / rho0, 0.3, 0.2, p, pold, rho_u, ua, va, wa)
+    oracle.correc(ng, 1, 1, dli, dzci, 0.3, rho0, p, ub, vb, wb)
+    for x, y in ((ua, ub), (va, vb), (wa, wb)):
+        assert relmax(x, y) < 1e-14
+    # (b) pold == p, dt == dto
+    ua, va, wa = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    ub, vb, wb = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    oracle.pres_tw_src(ng, dli, 1, 1, dzci, 1 / rho0, 0.3, 0.3, p, p, rho, ua, va, wa)
+    oracle.correc(ng, 1, 1, dli, dzci, 0.3, rho0, p, ub, vb, wb, rho=rho, const_coeff=False)
+    for x, y in ((ua, ub), (va, vb), (wa, wb)):
+        assert relmax(interior(x), interior(y)) < 1e-12
+    # (c) pres_sp_src == constant-coefficient correc with pold
+    ua, va, wa = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    ub, vb, wb = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F")
+    oracle.pres_sp_src(ng, 0.3, dli, 1, 1, dzci, 1 / 4.0, pold, ua, va, wa)
+    oracle.correc(ng, 1, 1, dli, dzci, 0.3, 4.0, pold, ub, vb, wb)
+    for x, y in ((ua, ub), (va, vb), (wa, wb)):
+        assert relmax(x, y) < 1e-14

@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the pressure Poisson solve on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+
+A step is one solve of the 512^3 stretched-z channel (periodic x and y,
+Neumann z walls, Lx = 2 pi, Ly = pi, tanh stretching 1.5) on a seeded N(0,1)
+right-hand side already resident in HBM.  The line reports ns per grid
+point for the plain FP64 solve (the `value`), the FP32 + FP64-refinement
+solve to a 1e-10 relative residual, per-kernel HBM roofline figures from
+HIP events on the library stream, and the FP64 CPU oracle timed on a
+bounded slab of the same workload on the host cores.
+
+Only one GPU is used: --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("ns/gridpoint pressure solve @512³: FP64 vs FP32+refine @1e-10; "
+          "% HBM peak")
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# Algorithmic HBM bytes per grid point of each pipeline stage: one read and
+# one write of the field at the stage's storage width.
+STAGE_BYTES_FP64 = {"x_fwd": 16, "y_fwd": 16, "z_solve": 16, "y_bwd": 16, "x_bwd": 16}
+SOLVE_BYTES_FP64 = 80
+
+
+def time_steps(fn, steps, warmup, api):
+    for _ in range(warmup):
+        fn()
+    api.synchronize()
+    e0, e1 = api.Event(), api.Event()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    api.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    return e0.elapsed_ms(e1) / steps, wall / steps
+
+
+def cpu_baseline(case, nz_sample=128, reps=2):
+    """The C FP64 oracle (solver_cpu restated) on a 512x512xnz_sample slab
+    with the same BCs and stretching; ns per grid point."""
+    from oracle import pyoracle as po
+    nx, ny, _ = case.ng
+    lx, ly, lz = case.lengths
+    ng = (nx, ny, nz_sample)
+    dzc, dzf, zc, zf = po.initgrid(nz_sample, case.alpha, lz, 1)
+    dli = (nx / lx, ny / ly, nz_sample / lz)
+    s = po.Solver(ng, dli, 1.0 / dzc, 1.0 / dzf, case.cbc)
+    rng = np.random.Generator(np.random.Philox(case.seed))
+    p0 = np.zeros((nx + 2, ny + 2, nz_sample + 2), order="F")
+    p0[1:-1, 1:-1, 1:-1] = rng.standard_normal((nx, ny, nz_sample))
+    best = None
+    for _ in range(reps):
+        p = p0.copy(order="F")
+        t0 = time.perf_counter()
+        s.solve(p)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    npts = nx * ny * nz_sample
+    return {"value": best * 1e9 / npts, "unit": "ns/gridpoint",
+            "cores": int(po.lib().or_max_threads()), "kind": "port",
+            "sample": f"C FP64 oracle (solver_cpu restated, own FFT) on a "
+                      f"{nx}x{ny}x{nz_sample} slab, {case.cbc}, stretch {case.alpha}; "
+                      f"best of {reps}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg2_512_channel")
+    ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-refine", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; run it with --gpus 1 "
+              "(the multi-GPU slab exchange is a design, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    from flutas_amd import api, workloads as wl
+    case = wl.CONFIGS[args.config]
+    nx, ny, nz = case.ng
+    npts = case.npoints
+    log(f"bench: {case.name} {case.ng} {case.cbc} stretch={case.alpha}")
+    dzc, dzf, dzci, dzfi = wl.z_grid(api, case)
+    f_host = wl.rhs_normal(case, dzf)
+    f = api.DeviceArray.from_host(f_host)
+    p = api.DeviceArray(f.shape)
+    p.zero()
+
+    # ---- FP64 ---------------------------------------------------------
+    plan = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc)
+    ms, wall_ms = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
+    relres64 = plan.residual(f, p)
+    log(f"bench: fp64 {ms:.3f} ms/solve (wall {wall_ms:.3f}), relres {relres64:.2e}")
+    scratch = api.DeviceArray(f.shape)
+    api.copy_interior(nx, ny, nz, f, scratch)
+    stages = plan.profile(scratch, args.profile_reps)
+    stage_rows = []
+    for name, t in stages:
+        b = STAGE_BYTES_FP64.get(name, 16) * npts
+        stage_rows.append({"stage": name, "ms": t, "GBps": b / (t * 1e6)})
+    dom = max(stage_rows, key=lambda r: r["ms"])
+    log("bench: stages " + ", ".join(f"{r['stage']} {r['ms']:.3f} ms ({r['GBps']:.0f} GB/s)"
+                                     for r in stage_rows))
+    solve_gbps = SOLVE_BYTES_FP64 * npts / (ms * 1e6)
+    plan.destroy()
+    del scratch
+
+    # ---- FP32 + FP64 refinement ---------------------------------------
+    refine = None
+    if not args.no_refine:
+        plan32 = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc,
+                                 precision=api.FP32_REFINE, tol=1e-10)
+        ms32, _ = time_steps(lambda: plan32.solve_into(f, p), max(3, args.steps // 2),
+                             args.warmup, api)
+        iters, rel = plan32.stats()
+        check = plan32.residual(f, p)
+        refine = {"ns_per_gridpoint": ms32 * 1e6 / npts, "ms_per_solve": ms32,
+                  "iterations": iters, "relres": rel, "relres_check": check,
+                  "throughput_vs_fp64": ms / ms32,
+                  "algorithmic_bytes_per_point": 48 + 96 * iters + 24}
+        log(f"bench: fp32+refine {ms32:.3f} ms/solve, {iters} iters, relres {rel:.2e}")
+        plan32.destroy()
+
+    result = {
+        "metric": METRIC,
+        "value": ms * 1e6 / npts,
+        "unit": "ns/gridpoint",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{case.name}: {nx}x{ny}x{nz} pressure Poisson solve, "
+                               f"BC {case.cbc}, Lx,Ly,Lz={tuple(round(v, 6) for v in case.lengths)}, "
+                               f"tanh z stretching {case.alpha}, seeded N(0,1) RHS "
+                               "(dz-weighted mean removed), inputs resident in HBM",
+                   "ng": list(case.ng), "bc": case.cbc},
+        "roofline": {"bound": "hbm", "kernel": dom["stage"],
+                     "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
+        "solve_roofline": {"bytes_per_point": SOLVE_BYTES_FP64, "achieved": solve_gbps,
+                           "frac": solve_gbps / HBM_PEAK_GBS},
+        "stages": stage_rows,
+        "fp64_relres": relres64,
+        "fp32_refine": refine,
+    }
+    if not args.no_cpu:
+        log("bench: cpu baseline (oracle) ...")
+        result["cpu_baseline"] = cpu_baseline(case)
+        log(f"bench: cpu {result['cpu_baseline']['value']:.2f} ns/pt on "
+            f"{result['cpu_baseline']['cores']} threads")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

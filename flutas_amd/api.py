@@ -1,0 +1,371 @@
+"""Python host mirror of the reference's projection-step interface, bound to
+libflutas_amd.so through its C-ABI (ctypes; no torch types).
+
+The functions keep the reference's names, argument order and meaning:
+
+    initsolver(n, dims, dims_z, dli, nh_d, dzci, dzfi, cbc, bc, c_or_f)
+        -> (lambdaxy, a, b, c, arrplan, normfft, rhsbx, rhsby, rhsbz)
+                                                (src/initsolver.f90:21)
+    solver_gpu(n, dims, arrplan, normfft, lambdaxy, a, b, c, bc, c_or_f, p)
+                                                (src/solver_gpu.f90:31)
+    fillps(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dzfi,dti,rho0,u,v,w,p)
+                                                (src/fillps.f90:16)
+    correc(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dzci,dt,rho0,p,u,v,w,rho)
+                                                (src/correc.f90:16)
+    pres_tw_src / pres_sp_src / chkdiv / boundp / updt_rhs_b / initgrid
+
+Field arguments are DeviceArray objects (Fortran-ordered device
+allocations with the halo included).  Failures raise FlutasError carrying
+the library's status code; a missing or unbuilt library raises at import
+time -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflutas_amd.so")
+
+FLUTAS_OK = 0
+FLUTAS_ENOMEM = 12
+FLUTAS_EINVAL = 22
+FLUTAS_EDEVICE = 1001
+FLUTAS_ENONFINITE = 1002
+FLUTAS_ENOTCONV = 1003
+FLUTAS_EUNSUPPORTED = 1004
+FP64 = 64
+FP32_REFINE = 32
+
+
+class FlutasError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_D = C.POINTER(C.c_double)
+_VP = C.c_void_p
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m flutas_amd.build` "
+            "(hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    lib.flutas_last_error.restype = C.c_char_p
+    lib.flutas_version.restype = C.c_char_p
+    lib.flutas_psolve_stage_name.restype = C.c_char_p
+    lib.flutas_psolve_stage_name.argtypes = [_VP, C.c_int]
+    for name in ("flutas_device_free", "flutas_psolve_destroy",
+                 "flutas_event_record", "flutas_event_destroy"):
+        getattr(lib, name).argtypes = [_VP]
+    lib.flutas_device_alloc.argtypes = [C.c_size_t, C.POINTER(_VP)]
+    lib.flutas_memcpy_h2d.argtypes = [_VP, _VP, C.c_size_t]
+    lib.flutas_memcpy_d2h.argtypes = [_VP, _VP, C.c_size_t]
+    lib.flutas_memset_d.argtypes = [_VP, C.c_int, C.c_size_t]
+    lib.flutas_set_stream.argtypes = [_VP]
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc):
+    if rc != FLUTAS_OK:
+        raise FlutasError(rc, lib.flutas_last_error().decode())
+
+
+def synchronize():
+    _check(lib.flutas_synchronize())
+
+
+class DeviceArray:
+    """A Fortran-ordered device allocation of a given shape and dtype."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        ptr = _VP()
+        _check(lib.flutas_device_alloc(C.c_size_t(self.nbytes), C.byref(ptr)))
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.asfortranarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.asfortranarray(a, dtype=self.dtype)
+        assert a.shape == self.shape
+        _check(lib.flutas_memcpy_h2d(_VP(self.ptr), a.ctypes.data_as(_VP),
+                                     C.c_size_t(self.nbytes)))
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        _check(lib.flutas_memcpy_d2h(out.ctypes.data_as(_VP), _VP(self.ptr),
+                                     C.c_size_t(self.nbytes)))
+        return out
+
+    def zero(self):
+        _check(lib.flutas_memset_d(_VP(self.ptr), 0, C.c_size_t(self.nbytes)))
+
+    def free(self):
+        if self.ptr:
+            lib.flutas_device_free(_VP(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def c(self):
+        return C.cast(C.c_void_p(self.ptr), _D)
+
+
+def _dptr(a):
+    if isinstance(a, DeviceArray):
+        return a.c
+    raise TypeError("field arguments must be DeviceArray objects")
+
+
+def _chars(s, n):
+    s = "".join(s) if not isinstance(s, str) else s
+    if len(s) != n:
+        raise ValueError(f"expected {n} characters, got {s!r}")
+    return C.c_char_p(s.encode())
+
+
+def _bc6(cbc):
+    """Accepts 'PPPPNN' or a (2,3)-shaped Fortran (0:1,3) array of chars."""
+    if isinstance(cbc, str):
+        return cbc
+    arr = np.asarray(cbc)
+    if arr.shape == (2, 3):
+        return "".join(arr[:, d][s] for d in range(3) for s in range(2))
+    return "".join(arr.ravel())
+
+
+class PoissonPlan:
+    """initsolver + fftini (src/initsolver.f90:21, src/fft.f90:24): owns
+    the device plan; `arrplan` of the reference is this object."""
+
+    def __init__(self, ng, dli, dzci, dzfi, cbc, bc=(0.0,) * 6, c_or_f="ccc",
+                 nh_d=1, precision=FP64, tol=1e-10):
+        self.ng = tuple(int(v) for v in ng)
+        self.cbc = _bc6(cbc)
+        self.nh_d = int(nh_d)
+        self.precision = int(precision)
+        dzci = np.ascontiguousarray(dzci, np.float64)
+        dzfi = np.ascontiguousarray(dzfi, np.float64)
+        if dzci.size < self.ng[2] + 2 * nh_d or dzfi.size < self.ng[2] + 2 * nh_d:
+            raise ValueError("dzci/dzfi must cover (1-nh_d : nz+nh_d)")
+        h = _VP()
+        ngc = (C.c_int * 3)(*self.ng)
+        bcv = (C.c_double * 6)(*[float(v) for v in np.asarray(bc, np.float64).ravel()])
+        dliv = (C.c_double * 3)(*[float(v) for v in dli])
+        _check(lib.flutas_psolve_create(
+            ngc, _chars(self.cbc, 6), bcv, _chars(c_or_f, 3), dliv,
+            C.c_int(nh_d), dzci.ctypes.data_as(_D), dzfi.ctypes.data_as(_D),
+            C.c_int(self.precision), C.c_double(tol), C.byref(h)))
+        self._h = h.value
+        nx, ny, nz = self.ng
+        self.lambdaxy = np.zeros((nx, ny), order="F")
+        self.a, self.b, self.c = np.zeros(nz), np.zeros(nz), np.zeros(nz)
+        self.rhsbx = np.zeros((ny, nz, 2), order="F")
+        self.rhsby = np.zeros((nx, nz, 2), order="F")
+        self.rhsbz = np.zeros((nx, ny, 2), order="F")
+        nf = C.c_double()
+        _check(lib.flutas_psolve_coeffs(
+            _VP(self._h), self.lambdaxy.ctypes.data_as(_D), self.a.ctypes.data_as(_D),
+            self.b.ctypes.data_as(_D), self.c.ctypes.data_as(_D), C.byref(nf),
+            self.rhsbx.ctypes.data_as(_D), self.rhsby.ctypes.data_as(_D),
+            self.rhsbz.ctypes.data_as(_D)))
+        self.normfft = nf.value
+
+    def solve(self, p):
+        """solver_gpu on the interior of a halo-1 DeviceArray (in place)."""
+        if p.shape != tuple(v + 2 for v in self.ng) or p.dtype != np.float64:
+            raise ValueError("p must be a float64 DeviceArray of shape ng+2")
+        _check(lib.flutas_psolve_solve(_VP(self._h), p.c))
+
+    def solve_into(self, f, p):
+        """Out-of-place solve: RHS from f's interior, solution into p's."""
+        for a in (f, p):
+            if a.shape != tuple(v + 2 for v in self.ng) or a.dtype != np.float64:
+                raise ValueError("f and p must be float64 DeviceArrays of shape ng+2")
+        _check(lib.flutas_psolve_solve_into(_VP(self._h), f.c, p.c))
+
+    def stats(self):
+        it, rr = C.c_int(), C.c_double()
+        _check(lib.flutas_psolve_stats(_VP(self._h), C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
+    def residual(self, f, p):
+        rr = C.c_double()
+        _check(lib.flutas_psolve_residual(_VP(self._h), f.c, p.c, C.byref(rr)))
+        return rr.value
+
+    def profile(self, p, reps):
+        ms = (C.c_double * 32)()
+        ns = C.c_int()
+        _check(lib.flutas_psolve_profile(_VP(self._h), p.c, C.c_int(reps),
+                                         C.c_int(32), ms, C.byref(ns)))
+        return [(lib.flutas_psolve_stage_name(_VP(self._h), s).decode(), ms[s])
+                for s in range(ns.value)]
+
+    def rhsb_device(self):
+        rx, ry, rz = _D(), _D(), _D()
+        _check(lib.flutas_psolve_rhsb_device(_VP(self._h), C.byref(rx),
+                                             C.byref(ry), C.byref(rz)))
+        return rx, ry, rz
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            lib.flutas_psolve_destroy(_VP(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---- reference-named entry points --------------------------------------
+
+def initsolver(n, dims, dims_z, dli, nh_d, dzci, dzfi, cbc, bc, c_or_f,
+               precision=FP64, tol=1e-10):
+    """src/initsolver.f90:21-44 at one GPU (dims = dims_z = (1,1,1))."""
+    if tuple(dims) != (1, 1, 1) and tuple(dims)[:2] != (1, 1):
+        raise FlutasError(FLUTAS_EUNSUPPORTED, "one GPU: dims must be 1")
+    plan = PoissonPlan(n, dli, dzci, dzfi, cbc, bc, c_or_f, nh_d, precision, tol)
+    return (plan.lambdaxy, plan.a, plan.b, plan.c, plan, plan.normfft,
+            plan.rhsbx, plan.rhsby, plan.rhsbz)
+
+
+def fftini(plan):  # the plan is created by initsolver/PoissonPlan
+    return plan
+
+
+def fftend(arrplan):
+    """src/fft.f90:159-179."""
+    arrplan.destroy()
+
+
+def solver_gpu(n, dims, arrplan, normfft, lambdaxy, a, b, c, bc, c_or_f, p):
+    """src/solver_gpu.f90:31-47: the coefficients live in the plan; they are
+    accepted for signature parity."""
+    arrplan.solve(p)
+
+
+def fillps(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w, p):
+    _check(lib.flutas_fillps(nx, ny, nz, nh_d, nh_u, C.c_double(dxi), C.c_double(dyi),
+                             C.c_double(dzi), _dptr(dzfi), C.c_double(dti),
+                             C.c_double(rho0), _dptr(u), _dptr(v), _dptr(w), _dptr(p)))
+
+
+def pres_tw_src(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, f_t12,
+                f_t12_o, p, pold, rho, u, v, w, const_coeff=True):
+    _check(lib.flutas_pres_tw_src(nx, ny, nz, C.c_double(dxi), C.c_double(dyi),
+                                  C.c_double(dzi), nh_d, nh_u, _dptr(dzci),
+                                  C.c_double(rho0i), C.c_double(f_t12),
+                                  C.c_double(f_t12_o), _dptr(p), _dptr(pold),
+                                  _dptr(rho), _dptr(u), _dptr(v), _dptr(w),
+                                  1 if const_coeff else 0))
+
+
+def pres_sp_src(nx, ny, nz, f_t12, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i,
+                pold, u, v, w):
+    _check(lib.flutas_pres_sp_src(nx, ny, nz, C.c_double(f_t12), C.c_double(dxi),
+                                  C.c_double(dyi), C.c_double(dzi), nh_d, nh_u,
+                                  _dptr(dzci), C.c_double(rho0i), _dptr(pold),
+                                  _dptr(u), _dptr(v), _dptr(w)))
+
+
+def correc(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, p, u, v, w,
+           rho=None, const_coeff=True):
+    rp = _dptr(rho) if rho is not None else _D()
+    _check(lib.flutas_correc(nx, ny, nz, nh_d, nh_u, C.c_double(dxi), C.c_double(dyi),
+                             C.c_double(dzi), _dptr(dzci), C.c_double(dt),
+                             C.c_double(rho0), _dptr(p), _dptr(u), _dptr(v),
+                             _dptr(w), rp, 1 if const_coeff else 0))
+
+
+def chkdiv(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w):
+    tot, mx = C.c_double(), C.c_double()
+    _check(lib.flutas_chkdiv(nx, ny, nz, C.c_double(dxi), C.c_double(dyi),
+                             C.c_double(dzi), nh_d, nh_u, _dptr(dzfi), _dptr(u),
+                             _dptr(v), _dptr(w), C.byref(tot), C.byref(mx)))
+    return tot.value, mx.value
+
+
+def boundp(cbc, n, bc, dl, dzc_0, dzc_n, p):
+    ngc = (C.c_int * 3)(*[int(v) for v in n])
+    bcv = (C.c_double * 6)(*[float(v) for v in np.asarray(bc, np.float64).ravel()])
+    dlv = (C.c_double * 3)(*[float(v) for v in dl])
+    _check(lib.flutas_boundp(_chars(_bc6(cbc), 6), ngc, bcv, dlv,
+                             C.c_double(dzc_0), C.c_double(dzc_n), _dptr(p)))
+
+
+def updt_rhs_b(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p):
+    """rhsb* are device pointers (PoissonPlan.rhsb_device()) or DeviceArrays."""
+    ptrs = [(_dptr(r) if isinstance(r, DeviceArray) else r) for r in (rhsbx, rhsby, rhsbz)]
+    _check(lib.flutas_updt_rhs_b(nx, ny, nz, _chars(c_or_f, 3), _chars(_bc6(cbc), 6),
+                                 *ptrs, _dptr(p)))
+
+
+def copy_interior(nx, ny, nz, src, dst):
+    _check(lib.flutas_copy_interior(nx, ny, nz, _dptr(src), _dptr(dst)))
+
+
+def add_interior(nx, ny, nz, pold, p):
+    _check(lib.flutas_add_interior(nx, ny, nz, _dptr(pold), _dptr(p)))
+
+
+R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "RODFT01": 8,
+            "RODFT10": 9}
+
+
+def fft_lines(kind, data, n, howmany, stride=1, dist=None):
+    """fft (src/fft.f90:181-193): in-place r2r on device lines."""
+    k = R2R_KIND[kind] if isinstance(kind, str) else int(kind)
+    dist = n if dist is None else dist
+    _check(lib.flutas_fft_lines(C.c_int(k), C.c_int(n), C.c_int(howmany),
+                                C.c_longlong(stride), C.c_longlong(dist), _dptr(data)))
+
+
+def initgrid(n, gr, lz, nh_d=1):
+    """src/initgrid.f90:17-97; returns dzc, dzf, zc, zf over (1-nh_d:n+nh_d)."""
+    arrs = [np.zeros(n + 2 * nh_d) for _ in range(4)]
+    _check(lib.flutas_initgrid(n, C.c_double(gr), C.c_double(lz), nh_d,
+                               *[a.ctypes.data_as(_D) for a in arrs]))
+    return arrs
+
+
+class Event:
+    def __init__(self):
+        e = _VP()
+        _check(lib.flutas_event_create(C.byref(e)))
+        self._e = e.value
+
+    def record(self):
+        _check(lib.flutas_event_record(_VP(self._e)))
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        _check(lib.flutas_event_elapsed_ms(_VP(self._e), _VP(end._e), C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib.flutas_event_destroy(_VP(self._e))
+        except Exception:
+            pass

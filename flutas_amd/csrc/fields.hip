@@ -1,0 +1,518 @@
+// Streaming kernels of the projection step around the Poisson solve:
+// fillps, pres_tw_src, pres_sp_src, correc, chkdiv, boundp, updt_rhs_b, the
+// increment bookkeeping, and the FP64 residual of the refined solver.
+// One lane per grid point, lanes adjacent in i (coalesced); every kernel is
+// a single HBM pass.  The arithmetic keeps the reference's operation order
+// (contraction off) so results match the FP64 CPU path to rounding.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace flutas {
+
+// index of (i,j,k) in a halo-h block whose first element is (1-h,1-h,1-h)
+__device__ __forceinline__ long long ixh(int i, int j, int k, int h, int nx,
+                                         int ny) {
+  return (long long)(i - 1 + h) +
+         (long long)(nx + 2 * h) *
+             ((long long)(j - 1 + h) + (long long)(ny + 2 * h) * (k - 1 + h));
+}
+
+static inline dim3 grid3(int nx, int ny, int nz, int bx) {
+  return dim3((nx + bx - 1) / bx, ny, nz);
+}
+
+// ---- fillps (src/fillps.f90:42-61, _CONSTANT_COEFFS_POISSON) -------------
+__global__ void __launch_bounds__(256)
+    k_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
+             const double *__restrict__ dzfi, double dti, double rho0,
+             const double *__restrict__ u, const double *__restrict__ v,
+             const double *__restrict__ w, double *__restrict__ p) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const double dtidxi = dti * dxi, dtidyi = dti * dyi;
+  const long long c = ixh(i, j, k, nh_u, nx, ny);
+  const double val =
+      ((w[c] - w[ixh(i, j, k - 1, nh_u, nx, ny)]) * dti * dzfi[k + nh_d - 1] +
+       (v[c] - v[ixh(i, j - 1, k, nh_u, nx, ny)]) * dtidyi) +
+      (u[c] - u[c - 1]) * dtidxi;
+  p[ixh(i, j, k, 1, nx, ny)] = val * rho0;
+}
+
+// ---- pres_tw_src (src/source.f90:272-306) -------------------------------
+__global__ void __launch_bounds__(256)
+    k_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi, int nh_d,
+                  int nh_u, const double *__restrict__ dzci, double rho0i,
+                  double f_t12, double f1, double f2,
+                  const double *__restrict__ p, const double *__restrict__ pold,
+                  const double *__restrict__ rho, double *__restrict__ u,
+                  double *__restrict__ v, double *__restrict__ w, int cc) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
+  const long long ip = c1 + 1, jp = c1 + (nx + 2),
+                  kp = c1 + (long long)(nx + 2) * (ny + 2);
+  const double rc = rho[c1];
+  const double rhoxi = 1.0 / (0.5 * (rho[ip] + rc));
+  const double rhoyi = 1.0 / (0.5 * (rho[jp] + rc));
+  const double rhozi = 1.0 / (0.5 * (rho[kp] + rc));
+  const double dzk = dzci[k + nh_d - 1];
+  const double pc = p[c1];
+  if (cc) {
+    const double hc = f1 * pc - f2 * pold[c1];
+    u[cu] = u[cu] + f_t12 * ((-((p[ip] - pc) * dxi)) * rho0i -
+                             (rhoxi - rho0i) * ((f1 * p[ip] - f2 * pold[ip]) - hc) * dxi);
+    v[cu] = v[cu] + f_t12 * ((-((p[jp] - pc) * dyi)) * rho0i -
+                             (rhoyi - rho0i) * ((f1 * p[jp] - f2 * pold[jp]) - hc) * dyi);
+    w[cu] = w[cu] + f_t12 * ((-((p[kp] - pc) * dzk)) * rho0i -
+                             (rhozi - rho0i) * ((f1 * p[kp] - f2 * pold[kp]) - hc) * dzk);
+  } else {
+    u[cu] = u[cu] + f_t12 * (-((p[ip] - pc) * dxi)) * rhoxi;
+    v[cu] = v[cu] + f_t12 * (-((p[jp] - pc) * dyi)) * rhoyi;
+    w[cu] = w[cu] + f_t12 * (-((p[kp] - pc) * dzk)) * rhozi;
+  }
+}
+
+// ---- pres_sp_src (src/source.f90:326-343) -------------------------------
+__global__ void __launch_bounds__(256)
+    k_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi, double dyi,
+                  int nh_d, int nh_u, const double *__restrict__ dzci,
+                  double rho0i, const double *__restrict__ pold,
+                  double *__restrict__ u, double *__restrict__ v,
+                  double *__restrict__ w) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
+  const double pc = pold[c1];
+  u[cu] = u[cu] + f_t12 * (-((pold[c1 + 1] - pc) * dxi)) * rho0i;
+  v[cu] = v[cu] + f_t12 * (-((pold[c1 + (nx + 2)] - pc) * dyi)) * rho0i;
+  w[cu] = w[cu] + f_t12 * (-((pold[c1 + (long long)(nx + 2) * (ny + 2)] - pc) *
+                             dzci[k + nh_d - 1])) * rho0i;
+}
+
+// ---- correc (src/correc.f90:120-144) ------------------------------------
+__global__ void __launch_bounds__(256)
+    k_correc(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
+             double factorj, const double *__restrict__ dzci, double dt,
+             double rho0i, const double *__restrict__ p, double *__restrict__ u,
+             double *__restrict__ v, double *__restrict__ w,
+             const double *__restrict__ rho, int cc) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
+  const long long ip = c1 + 1, jp = c1 + (nx + 2),
+                  kp = c1 + (long long)(nx + 2) * (ny + 2);
+  const double pc = p[c1];
+  const double fz = dt * dzci[k + nh_d - 1];
+  if (cc) {
+    u[cu] = u[cu] - factori * (p[ip] - pc) * rho0i;
+    v[cu] = v[cu] - factorj * (p[jp] - pc) * rho0i;
+    w[cu] = w[cu] - fz * (p[kp] - pc) * rho0i;
+  } else {
+    const double rc = rho[c1];
+    const double rhox = 0.5 * (rho[ip] + rc), rhoy = 0.5 * (rho[jp] + rc),
+                 rhoz = 0.5 * (rho[kp] + rc);
+    u[cu] = u[cu] - factori * (p[ip] - pc) / rhox;
+    v[cu] = v[cu] - factorj * (p[jp] - pc) / rhoy;
+    w[cu] = w[cu] - fz * (p[kp] - pc) / rhoz;
+  }
+}
+
+// ---- chkdiv (src/chkdiv.f90:38-60): per-block partial sum and max -------
+constexpr int kRedThreads = 256;
+
+__global__ void __launch_bounds__(kRedThreads)
+    k_chkdiv(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
+             const double *__restrict__ dzfi, const double *__restrict__ u,
+             const double *__restrict__ v, const double *__restrict__ w,
+             double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double ssum[kRedThreads], smax[kRedThreads];
+  const long long total = (long long)nx * ny * nz;
+  double s = 0.0, m = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % nx) + 1;
+    const long long r = t / nx;
+    const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
+    const long long c = ixh(i, j, k, nh_u, nx, ny);
+    const double div =
+        ((w[c] - w[ixh(i, j, k - 1, nh_u, nx, ny)]) * dzfi[k + nh_d - 1] +
+         (v[c] - v[ixh(i, j - 1, k, nh_u, nx, ny)]) * dyi) +
+        (u[c] - u[c - 1]) * dxi;
+    s += div;
+    const double ad = fabs(div);
+    m = (ad > m || ad != ad) ? ad : m;
+  }
+  ssum[threadIdx.x] = s;
+  smax[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + off];
+      const double o = smax[threadIdx.x + off];
+      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ssum[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// ---- boundp / set_bc, nh_p = 1 (src/bound.f90:146-646) -------------------
+// One face: dir 0/1/2, side 0/1; type 'P' periodic copy, 'D' centred
+// Dirichlet p0 = 2v - p1, 'N' p0 = p1 -+ dr v.  Loops over the full
+// tangential extent including the halo, as set_bc does.
+__global__ void k_set_bc(int nx, int ny, int nz, char ctype, int side, int dir,
+                         double value, double dr, double *__restrict__ p) {
+  const int n[3] = {nx, ny, nz};
+  const long long st[3] = {1, nx + 2, (long long)(nx + 2) * (ny + 2)};
+  const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+  const int b1 = blockIdx.x * blockDim.x + threadIdx.x, b2 = blockIdx.y;
+  if (b1 > n[t1] + 1 || b2 > n[t2] + 1) return;
+  const long long base = b1 * st[t1] + b2 * st[t2], s = st[dir];
+  const int nd = n[dir];
+  if (ctype == 'P') {
+    if (side == 0) p[base] = p[base + nd * s];
+    else p[base + (nd + 1) * s] = p[base + s];
+    return;
+  }
+  double factor = value, sgn = 0.0;
+  if (ctype == 'D') { factor = 2.0 * factor; sgn = -1.0; }
+  if (ctype == 'N') { factor = (side == 0) ? -dr * factor : dr * factor; sgn = 1.0; }
+  if (side == 0) p[base] = factor + sgn * p[base + s];
+  else p[base + (nd + 1) * s] = factor + sgn * p[base + nd * s];
+}
+
+// ---- updt_rhs_b (src/bound.f90:829-944) ------------------------------------
+__global__ void k_add_face(int nx, int ny, int nz, int dir, int q,
+                           const double *__restrict__ rhs,
+                           double *__restrict__ p) {
+  const int n[3] = {nx, ny, nz};
+  const long long st[3] = {1, nx + 2, (long long)(nx + 2) * (ny + 2)};
+  const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x, bb = blockIdx.y;
+  if (a >= n[t1] || bb >= n[t2]) return;
+  const long long base = (a + 1) * st[t1] + (bb + 1) * st[t2];
+  const long long m = a + (long long)n[t1] * bb, plane = (long long)n[t1] * n[t2];
+  p[base + 1 * st[dir]] += rhs[m];
+  p[base + (n[dir] - q) * st[dir]] += rhs[m + plane];
+}
+
+// ---- interior copies ------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_interior(int nx, int ny, int nz, const double *__restrict__ src,
+               double *__restrict__ dst, int add) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const long long c = ixh(i, j, k, 1, nx, ny);
+  dst[c] = add ? src[c] + dst[c] : src[c];
+}
+
+// ---- residual r = f - L p (refined solver) ------------------------------
+// L is the operator the spectral solve inverts: second differences in x, y
+// with homogeneous ghost rules (P wrap, N p0 = p1, D p0 = -p1) and the
+// tridiagonal rows a, b, c in z (periodic wrap for PP).
+template <typename Tr>
+__global__ void __launch_bounds__(256)
+    k_residual(OperatorDesc op, const double *__restrict__ f,
+               const double *__restrict__ p, Tr *__restrict__ r,
+               double *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double smax[256];
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  double am = 0.0;
+  if (i < nx) {
+    const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
+    const long long c = (i + 1) + sy * (j + 1) + sz * (k + 1);
+    const double pc = p[c];
+    double pm, pp;
+    if (i > 0) pm = p[c - 1];
+    else pm = (op.cbc[0] == 'P') ? p[c + (nx - 1)] : (op.cbc[0] == 'N') ? pc : -pc;
+    if (i < nx - 1) pp = p[c + 1];
+    else pp = (op.cbc[1] == 'P') ? p[c - (nx - 1)] : (op.cbc[1] == 'N') ? pc : -pc;
+    const double lx = (pp - 2.0 * pc + pm) * op.dx2;
+    if (j > 0) pm = p[c - sy];
+    else pm = (op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : (op.cbc[2] == 'N') ? pc : -pc;
+    if (j < ny - 1) pp = p[c + sy];
+    else pp = (op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : (op.cbc[3] == 'N') ? pc : -pc;
+    const double ly = (pp - 2.0 * pc + pm) * op.dy2;
+    double lz = op.b[k] * pc;
+    if (k > 0) lz += op.a[k] * p[c - sz];
+    else if (op.cbc[4] == 'P') lz += op.a[k] * p[c + sz * (nz - 1)];
+    if (k < nz - 1) lz += op.c[k] * p[c + sz];
+    else if (op.cbc[5] == 'P') lz += op.c[k] * p[c - sz * (nz - 1)];
+    const double res = f[c] - ((lx + ly) + lz);
+    if (r) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
+    am = fabs(res);
+    if (res != res) am = res;
+  }
+  smax[threadIdx.x] = am;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double o = smax[threadIdx.x + off];
+      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    part[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = smax[0];
+}
+
+__global__ void __launch_bounds__(256)
+    k_absmax(const double *__restrict__ f, int nx, int ny, int nz,
+             double *__restrict__ part) {
+  __shared__ double smax[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  double am = 0.0;
+  if (i < nx) {
+    const double v = f[(i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1))];
+    am = fabs(v);
+    if (v != v) am = v;
+  }
+  smax[threadIdx.x] = am;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double o = smax[threadIdx.x + off];
+      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    part[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = smax[0];
+}
+
+__global__ void __launch_bounds__(256)
+    k_accumulate(const float *__restrict__ r, double *__restrict__ p, int nx,
+                 int ny) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const long long c = (i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1));
+  p[c] = p[c] + (double)r[i + (long long)nx * (j + (long long)ny * k)];
+}
+
+int launch_accumulate(const float *r, double *p, int nx, int ny, int nz,
+                      hipStream_t s) {
+  const int bx = 256;
+  hipLaunchKernelGGL(k_accumulate, dim3((nx + bx - 1) / bx, ny, nz), dim3(bx), 0, s, r, p, nx, ny);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+static int xblock(int nx) { return std::min(256, ((nx + 63) / 64) * 64); }
+
+template <typename Tr>
+int launch_residual(const OperatorDesc &op, const double *f, const double *p,
+                    Tr *r, double *partial, int *nparts, hipStream_t s) {
+  const int bx = 256;
+  dim3 grid((op.nx + bx - 1) / bx, op.ny, op.nz);
+  hipLaunchKernelGGL(k_residual<Tr>, grid, dim3(bx), 0, s, op, f, p, r, partial);
+  FL_HIP(hipGetLastError());
+  *nparts = (int)(grid.x * grid.y * grid.z);
+  return FLUTAS_OK;
+}
+template int launch_residual<float>(const OperatorDesc &, const double *, const double *, float *, double *, int *, hipStream_t);
+template int launch_residual<double>(const OperatorDesc &, const double *, const double *, double *, double *, int *, hipStream_t);
+
+int launch_absmax(const double *f, int nx, int ny, int nz, double *partial,
+                  int *nparts, hipStream_t s) {
+  const int bx = 256;
+  dim3 grid((nx + bx - 1) / bx, ny, nz);
+  hipLaunchKernelGGL(k_absmax, grid, dim3(bx), 0, s, f, nx, ny, nz, partial);
+  FL_HIP(hipGetLastError());
+  *nparts = (int)(grid.x * grid.y * grid.z);
+  return FLUTAS_OK;
+}
+
+}  // namespace flutas
+
+using namespace flutas;
+
+static int check_dims(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return fail(FLUTAS_EINVAL, "grid sizes must be positive");
+  return FLUTAS_OK;
+}
+
+extern "C" {
+
+int flutas_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzfi, double dti,
+                  double rho0, const double *u, const double *v,
+                  const double *w, double *p) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzfi || !u || !v || !w || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_fillps, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi,
+                     dti, rho0, u, v, w, p);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi,
+                       double dzi, int nh_d, int nh_u, const double *dzci,
+                       double rho0i, double f_t12, double f_t12_o,
+                       const double *p, const double *pold, const double *rho,
+                       double *u, double *v, double *w, int const_coeff) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzci || !p || !pold || !rho || !u || !v || !w) return fail(FLUTAS_EINVAL, "null pointer");
+  const double f1 = 1.0 + (f_t12 / f_t12_o), f2 = (f_t12 / f_t12_o);
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_pres_tw_src, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, dxi, dyi, nh_d, nh_u, dzci,
+                     rho0i, f_t12, f1, f2, p, pold, rho, u, v, w,
+                     const_coeff ? 1 : 0);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi,
+                       double dyi, double dzi, int nh_d, int nh_u,
+                       const double *dzci, double rho0i, const double *pold,
+                       double *u, double *v, double *w) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzci || !pold || !u || !v || !w) return fail(FLUTAS_EINVAL, "null pointer");
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_pres_sp_src, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, f_t12, dxi, dyi, nh_d, nh_u,
+                     dzci, rho0i, pold, u, v, w);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_correc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzci, double dt,
+                  double rho0, const double *p, double *u, double *v,
+                  double *w, const double *rho, int const_coeff) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzci || !p || !u || !v || !w || (!const_coeff && !rho))
+    return fail(FLUTAS_EINVAL, "null pointer");
+  const double rho0i = 1.0 / rho0, factori = dt * dxi, factorj = dt * dyi;
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_correc, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, nh_d, nh_u, factori, factorj,
+                     dzci, dt, rho0i, p, u, v, w, rho, const_coeff ? 1 : 0);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
+                  int nh_d, int nh_u, const double *dzfi, const double *u,
+                  const double *v, const double *w, double *divtot,
+                  double *divmax) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!dzfi || !u || !v || !w || !divtot || !divmax) return fail(FLUTAS_EINVAL, "null pointer");
+  const long long total = (long long)nx * ny * nz;
+  const int nb = (int)std::min<long long>(2048, (total + kRedThreads - 1) / kRedThreads);
+  double *part = nullptr;
+  FL_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * nb, current_stream()));
+  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(kRedThreads), 0, current_stream(),
+                     nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, part);
+  FL_HIP(hipGetLastError());
+  std::vector<double> h(2 * nb);
+  FL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb,
+                        hipMemcpyDeviceToHost, current_stream()));
+  FL_HIP(hipFreeAsync(part, current_stream()));
+  FL_HIP(hipStreamSynchronize(current_stream()));
+  double s = 0.0, m = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s += h[2 * b];
+    const double o = h[2 * b + 1];
+    if (o > m || o != o) m = o;
+  }
+  *divtot = s;
+  *divmax = m;
+  return FLUTAS_OK;
+}
+
+int flutas_boundp(const char cbc[6], const int n[3], const double bc[6],
+                  const double dl[3], double dzc_0, double dzc_n, double *p) {
+  if (!cbc || !n || !bc || !dl || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  for (int q = 0; q < 6; ++q)
+    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N')
+      return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+  hipStream_t s = current_stream();
+  const int nx = n[0], ny = n[1], nz = n[2];
+  auto face = [&](char t, int side, int dir, double value, double dr) -> int {
+    const int ext[3] = {nx + 2, ny + 2, nz + 2};
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    dim3 grid((ext[t1] + 255) / 256, ext[t2]);
+    hipLaunchKernelGGL(k_set_bc, grid, dim3(256), 0, s, nx, ny, nz, t, side,
+                       dir, value, dr, p);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  // periodic y and z halos first (the self-exchange of updthalo at one
+  // rank, src/bound.f90:182-183), then set_bc for x, y, z
+  if (cbc[2] == 'P') { FL_TRY(face('P', 0, 1, 0, 0)); FL_TRY(face('P', 1, 1, 0, 0)); }
+  if (cbc[4] == 'P') { FL_TRY(face('P', 0, 2, 0, 0)); FL_TRY(face('P', 1, 2, 0, 0)); }
+  FL_TRY(face(cbc[0], 0, 0, bc[0], dl[0]));
+  FL_TRY(face(cbc[1], 1, 0, bc[1], dl[0]));
+  if (cbc[2] != 'P') { FL_TRY(face(cbc[2], 0, 1, bc[2], dl[1])); FL_TRY(face(cbc[3], 1, 1, bc[3], dl[1])); }
+  if (cbc[4] != 'P') { FL_TRY(face(cbc[4], 0, 2, bc[4], dzc_0)); FL_TRY(face(cbc[5], 1, 2, bc[5], dzc_n)); }
+  return FLUTAS_OK;
+}
+
+int flutas_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
+                      const char cbc[6], const double *rhsbx,
+                      const double *rhsby, const double *rhsbz, double *p) {
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!c_or_f || !cbc || !rhsbx || !rhsby || !rhsbz || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  const int q1 = (c_or_f[0] == 'f' && cbc[1] == 'D') ? 1 : 0;
+  const int q2 = (c_or_f[1] == 'f' && cbc[3] == 'D') ? 1 : 0;
+  const int q3 = (c_or_f[2] == 'f' && cbc[5] == 'D') ? 1 : 0;
+  hipStream_t s = current_stream();
+  hipLaunchKernelGGL(k_add_face, dim3((ny + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 0, q1, rhsbx, p);
+  hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 1, q2, rhsby, p);
+  hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, ny), dim3(256), 0, s, nx, ny, nz, 2, q3, rhsbz, p);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_copy_interior(int nx, int ny, int nz, const double *src, double *dst) {
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!src || !dst) return fail(FLUTAS_EINVAL, "null pointer");
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_interior, grid3(nx, ny, nz, bx), dim3(bx), 0, current_stream(), nx, ny, nz, src, dst, 0);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_add_interior(int nx, int ny, int nz, const double *pold, double *p) {
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!pold || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_interior, grid3(nx, ny, nz, bx), dim3(bx), 0, current_stream(), nx, ny, nz, pold, p, 1);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+}  // extern "C"

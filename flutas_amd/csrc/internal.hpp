@@ -1,0 +1,114 @@
+// Internal declarations shared by the HIP translation units of
+// libflutas_amd.so.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/flutas_amd.h"
+
+namespace flutas {
+
+// ---- error handling ----------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define FL_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::flutas::fail(FLUTAS_EDEVICE, std::string(#call) + ": " +       \
+                                                hipGetErrorString(e_));       \
+  } while (0)
+
+#define FL_TRY(call)                                                          \
+  do {                                                                        \
+    int r_ = (call);                                                          \
+    if (r_ != FLUTAS_OK) return r_;                                           \
+  } while (0)
+
+hipStream_t current_stream();
+int check_device();  // EDEVICE unless a gfx950 device is current
+
+// ---- r2r line transforms -------------------------------------------------
+// Transform kinds (FFTW numbering, src/fftw.f90:38-61).
+enum Kind {
+  K_R2HC = 0, K_HC2R = 1,
+  K_REDFT01 = 4, K_REDFT10 = 5, K_RODFT01 = 8, K_RODFT10 = 9
+};
+
+constexpr int kMaxStages = 16;
+
+// Device-side plan of one real line length n: a complex Stockham FFT of
+// length N = n/2 (radix list), and the twiddle tables of the real split and
+// of the cosine/sine pre/post phases.
+template <typename T>
+struct LinePlan {
+  int n, N, nst;
+  int radix[kMaxStages];
+  const T *tw;   // 2N values: exp(-2 pi i m / N), m < N (re, im)
+  const T *twh;  // 2(N+1):  exp(-2 pi i k / n), k <= N
+  const T *twq;  // 2(n+1):  exp(-i pi k / (2n)), k <= n
+};
+
+// Host-owned device tables for one length, in double and float.
+struct LineTables {
+  int n = 0, N = 0, nst = 0;
+  int radix[kMaxStages] = {};
+  double *d_tw = nullptr, *d_twh = nullptr, *d_twq = nullptr;
+  float *f_tw = nullptr, *f_twh = nullptr, *f_twq = nullptr;
+  LinePlan<double> dplan() const;
+  LinePlan<float> fplan() const;
+  int build(int n);
+  void release();
+};
+
+// Geometry of a batch of lines: element e of line (l1,l2) lives at
+// base + l1*s1 + l2*s2 + e*se (separately for input and output).
+struct LineGeom {
+  int n;              // line length
+  int L1, L2;         // line counts
+  long long is1, is2, ise;
+  long long os1, os2, ose;
+};
+
+// Launch one batched line transform: in -> out (may alias only if the
+// geometries are identical).  Tin/Tout storage, T arithmetic.
+template <typename T, typename Tin, typename Tout>
+int launch_lines(const LineTables &tab, int kind, const Tin *in, Tout *out,
+                 const LineGeom &g, T scale, hipStream_t s);
+
+// ---- z tridiagonal solves ------------------------------------------------
+template <typename T>
+int launch_thomas(T *W, double *D, const double *lamx, const double *lamy,
+                  const double *a, const double *b, const double *c, int nx,
+                  int ny, int nz, int pin, hipStream_t s);
+template <typename T>
+int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
+                           const double *lamy, const double *a,
+                           const double *b, const double *c, int nx, int ny,
+                           int nz, int pin, hipStream_t s);
+
+// ---- refinement helpers ----------------------------------------------------
+struct OperatorDesc {
+  int nx, ny, nz;
+  char cbc[6];
+  double dx2, dy2;               // dli^2
+  const double *a, *b, *c;       // device z coefficients
+};
+// r = f - L p on the interior (halo-1 arrays f, p; dense r of type Tr),
+// and per-block max|r| partials (nblocks returned through *nparts).
+template <typename Tr>
+int launch_residual(const OperatorDesc &op, const double *f, const double *p,
+                    Tr *r, double *partial, int *nparts, hipStream_t s);
+// p(interior) += r (dense FP32 correction from the refined solve)
+int launch_accumulate(const float *r, double *p_halo, int nx, int ny, int nz,
+                      hipStream_t s);
+int launch_absmax(const double *f_halo, int nx, int ny, int nz, double *partial,
+                  int *nparts, hipStream_t s);
+
+}  // namespace flutas

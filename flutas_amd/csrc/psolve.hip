@@ -1,0 +1,653 @@
+// Poisson plan (initsolver/fftini restated on the host) and the solve
+// pipeline, plus the C-ABI plumbing (errors, stream, device memory).
+//
+// Pipeline of one FP64 solve on a one-GPU pencil (src/solver_gpu.f90:
+// 107-229 replaced; the reference's transposes become strided line
+// geometries, no copies):
+//   x-forward  p(interior, halo layout) -> W        r2r along x
+//   y-forward  W -> W                               r2r along y
+//   z-solve    W -> W                               Thomas per (i,j)
+//   y-backward W -> W
+//   x-backward W -> p(interior) * normfft
+// The refined mode runs the same chain in FP32 storage/arithmetic (the z
+// elimination stays FP64) and corrects with FP64 residuals r = f - L p
+// until max|r| / max|f| <= tol.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace flutas {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+static hipStream_t g_stream = nullptr;
+hipStream_t current_stream() { return g_stream; }
+
+int check_device() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(FLUTAS_EDEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return fail(FLUTAS_EDEVICE, hipGetErrorString(e));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(FLUTAS_EDEVICE, std::string("libflutas_amd is built for gfx950, device is ") + prop.gcnArchName);
+  return FLUTAS_OK;
+}
+
+// ---- initsolver restated (src/initsolver.f90) ------------------------------
+// pi as the reference defines it: acos(-1._rp) (param.f90:99)
+static double ref_pi() { return std::acos(-1.0); }
+
+// eigenvalues, 'c' branch (src/initsolver.f90:122-186, CPU ordering)
+static void eigen_c(int n, char b0, char b1, std::vector<double> &lam) {
+  const double pi = ref_pi();
+  lam.assign(n, 0.0);
+  for (int l = 1; l <= n; ++l) {
+    double s = 0.0;
+    if (b0 == 'P' && b1 == 'P') s = std::sin((1.0 * (l - 1)) * pi / (1.0 * n));
+    else if (b0 == 'N' && b1 == 'N') s = std::sin((1.0 * (l - 1)) * pi / (2.0 * n));
+    else if (b0 == 'D' && b1 == 'D') s = std::sin((1.0 * (l - 0)) * pi / (2.0 * n));
+    else s = std::sin((1.0 * (2 * l - 1)) * pi / (4.0 * n));
+    lam[l - 1] = -4.0 * (s * s);
+  }
+}
+
+// tridmatrix, 'c' (src/initsolver.f90:188-246); zci[k] == dzci(k)
+static void tridmatrix_c(char b0, char b1, int n, const double *zci,
+                         const double *zfi, std::vector<double> &a,
+                         std::vector<double> &b, std::vector<double> &c) {
+  a.resize(n); b.resize(n); c.resize(n);
+  for (int k = 1; k <= n; ++k) {
+    a[k - 1] = zfi[k] * zci[k - 1];
+    c[k - 1] = zfi[k] * zci[k];
+  }
+  for (int k = 0; k < n; ++k) b[k] = -(a[k] + c[k]);
+  auto fac = [](char t) { return t == 'P' ? 0.0 : (t == 'D' ? -1.0 : 1.0); };
+  b[0] = b[0] + fac(b0) * a[0];
+  b[n - 1] = b[n - 1] + fac(b1) * c[n - 1];
+}
+
+// bc_rhs, 'c' (src/initsolver.f90:248-298)
+static void bc_rhs_c(const char *cbc, const double *bc, const double dlc[2],
+                     const double dlf[2], long plane, std::vector<double> &rhs) {
+  rhs.assign(2 * plane, 0.0);
+  for (int ib = 0; ib < 2; ++ib) {
+    double factor = 0.0;
+    const double sgn = (ib == 0) ? 1.0 : -1.0;
+    if (cbc[ib] == 'D') factor = -2.0 * bc[ib];
+    else if (cbc[ib] == 'N') factor = sgn * dlc[ib] * bc[ib];
+    const double v = factor / dlc[ib] / dlf[ib];
+    std::fill(rhs.begin() + ib * plane, rhs.begin() + (ib + 1) * plane, v);
+  }
+}
+
+// find_fft, 'c' (src/fft.f90:242-264)
+static int find_fft_c(char b0, char b1, int &kf, int &kb, double &norm0) {
+  if (b0 == 'P' && b1 == 'P') { kf = K_R2HC; kb = K_HC2R; norm0 = 1; return 0; }
+  if (b0 == 'N' && b1 == 'N') { kf = K_REDFT10; kb = K_REDFT01; norm0 = 2; return 0; }
+  if (b0 == 'D' && b1 == 'D') { kf = K_RODFT10; kb = K_RODFT01; norm0 = 2; return 0; }
+  return -1;
+}
+
+}  // namespace flutas
+
+using namespace flutas;
+
+struct flutas_psolve {
+  int nx, ny, nz;
+  char cbc[6];
+  double bc[6];
+  double dli[3];
+  int precision;
+  double tol;
+  int max_iter = 12;
+  int kinds[4];
+  double normfft;
+  std::vector<double> lamx, lamy, a, b, c, rhsbx, rhsby, rhsbz;
+  LineTables tx, ty;
+  double *d_lamx = nullptr, *d_lamy = nullptr, *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
+  double *d_rhsbx = nullptr, *d_rhsby = nullptr, *d_rhsbz = nullptr;
+  double *W64 = nullptr;   // spectral work field (FP64 mode)
+  float *W32 = nullptr;    // spectral work field (refined mode)
+  double *D = nullptr;     // z elimination factors
+  double *P2 = nullptr;    // periodic-z second solution
+  double *F = nullptr;     // copy of the right-hand side (refined mode)
+  float *R32 = nullptr;    // residual (refined mode)
+  double *part = nullptr;  // reduction partials
+  size_t npart = 0;
+  int last_iters = 0;
+  double last_relres = -1.0;
+  // stage timing (flutas_psolve_profile)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> stage_names;
+  int nmark = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(T **d, const std::vector<T> &h) {
+  FL_HIP(hipMalloc((void **)d, std::max<size_t>(1, h.size()) * sizeof(T)));
+  if (!h.empty())
+    FL_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FLUTAS_OK;
+}
+
+void release(flutas_psolve *h) {
+  for (void *p : {(void *)h->d_lamx, (void *)h->d_lamy, (void *)h->d_a,
+                  (void *)h->d_b, (void *)h->d_c, (void *)h->d_rhsbx,
+                  (void *)h->d_rhsby, (void *)h->d_rhsbz, (void *)h->W64,
+                  (void *)h->W32, (void *)h->D, (void *)h->P2, (void *)h->F,
+                  (void *)h->R32, (void *)h->part})
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  h->ev.clear();
+  h->tx.release();
+  h->ty.release();
+}
+
+// Geometry helpers: the halo-1 field (0:nx+1,0:ny+1,0:nz+1) and the dense
+// (nx,ny,nz) work field.
+struct Geo {
+  long long hx, hxy, interior;  // halo strides and offset of (1,1,1)
+};
+Geo halo_geo(const flutas_psolve *h) {
+  Geo g;
+  g.hx = h->nx + 2;
+  g.hxy = (long long)(h->nx + 2) * (h->ny + 2);
+  g.interior = 1 + g.hx + g.hxy;
+  return g;
+}
+
+LineGeom geom_x_from_halo(const flutas_psolve *h) {
+  Geo q = halo_geo(h);
+  LineGeom g;
+  g.n = h->nx; g.L1 = h->ny; g.L2 = h->nz;
+  g.is1 = q.hx; g.is2 = q.hxy; g.ise = 1;
+  g.os1 = h->nx; g.os2 = (long long)h->nx * h->ny; g.ose = 1;
+  return g;
+}
+LineGeom geom_x_dense(const flutas_psolve *h) {
+  LineGeom g;
+  g.n = h->nx; g.L1 = h->ny; g.L2 = h->nz;
+  g.is1 = h->nx; g.is2 = (long long)h->nx * h->ny; g.ise = 1;
+  g.os1 = g.is1; g.os2 = g.is2; g.ose = 1;
+  return g;
+}
+LineGeom geom_x_to_halo(const flutas_psolve *h) {
+  Geo q = halo_geo(h);
+  LineGeom g;
+  g.n = h->nx; g.L1 = h->ny; g.L2 = h->nz;
+  g.is1 = h->nx; g.is2 = (long long)h->nx * h->ny; g.ise = 1;
+  g.os1 = q.hx; g.os2 = q.hxy; g.ose = 1;
+  return g;
+}
+LineGeom geom_y(const flutas_psolve *h) {
+  LineGeom g;
+  g.n = h->ny; g.L1 = h->nx; g.L2 = h->nz;
+  g.is1 = 1; g.is2 = (long long)h->nx * h->ny; g.ise = h->nx;
+  g.os1 = 1; g.os2 = g.is2; g.ose = h->nx;
+  return g;
+}
+
+// Marks the end of a pipeline stage when profiling.
+int mark(flutas_psolve *h, const char *name, hipStream_t s) {
+  if (!h->profiling) return FLUTAS_OK;
+  const size_t m = (size_t)h->nmark;
+  if (h->ev.size() < m + 2) {
+    hipEvent_t e;
+    FL_HIP(hipEventCreate(&e));
+    h->ev.push_back(e);
+    if (h->ev.size() < m + 2) {
+      FL_HIP(hipEventCreate(&e));
+      h->ev.push_back(e);
+    }
+  }
+  if (h->stage_names.size() < m + 1) h->stage_names.push_back(name);
+  else h->stage_names[m] = name;
+  FL_HIP(hipEventRecord(h->ev[m + 1], s));
+  ++h->nmark;
+  return FLUTAS_OK;
+}
+
+template <typename T>
+int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
+  const bool per = (h->cbc[4] == 'P' && h->cbc[5] == 'P');
+  const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
+  if (per)
+    return launch_thomas_periodic<T>(W, h->D, h->P2, h->d_lamx, h->d_lamy, h->d_a,
+                                     h->d_b, h->d_c, h->nx, h->ny, h->nz, pin, s);
+  return launch_thomas<T>(W, h->D, h->d_lamx, h->d_lamy, h->d_a, h->d_b, h->d_c,
+                          h->nx, h->ny, h->nz, pin, s);
+}
+
+int solve_fp64(flutas_psolve *h, const double *f, double *p, hipStream_t s) {
+  const Geo q = halo_geo(h);
+  double *pin = p + q.interior;
+  const double *fin = f + q.interior;
+  FL_TRY((launch_lines<double, double, double>(h->tx, h->kinds[0], fin, h->W64, geom_x_from_halo(h), 1.0, s)));
+  FL_TRY(mark(h, "x_fwd", s));
+  FL_TRY((launch_lines<double, double, double>(h->ty, h->kinds[2], h->W64, h->W64, geom_y(h), 1.0, s)));
+  FL_TRY(mark(h, "y_fwd", s));
+  FL_TRY(zsolve<double>(h, h->W64, s));
+  FL_TRY(mark(h, "z_solve", s));
+  FL_TRY((launch_lines<double, double, double>(h->ty, h->kinds[3], h->W64, h->W64, geom_y(h), 1.0, s)));
+  FL_TRY(mark(h, "y_bwd", s));
+  FL_TRY((launch_lines<double, double, double>(h->tx, h->kinds[1], h->W64, pin, geom_x_to_halo(h), h->normfft, s)));
+  FL_TRY(mark(h, "x_bwd", s));
+  h->last_iters = 0;
+  h->last_relres = -1.0;
+  return FLUTAS_OK;
+}
+
+OperatorDesc op_desc(const flutas_psolve *h) {
+  OperatorDesc op;
+  op.nx = h->nx; op.ny = h->ny; op.nz = h->nz;
+  std::memcpy(op.cbc, h->cbc, 6);
+  op.dx2 = h->dli[0] * h->dli[0];
+  op.dy2 = h->dli[1] * h->dli[1];
+  op.a = h->d_a; op.b = h->d_b; op.c = h->d_c;
+  return op;
+}
+
+int reduce_max(flutas_psolve *h, int nparts, hipStream_t s, double *out) {
+  std::vector<double> buf(nparts);
+  FL_HIP(hipMemcpyAsync(buf.data(), h->part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));
+  FL_HIP(hipStreamSynchronize(s));
+  double m = 0.0;
+  for (double v : buf)
+    if (v > m || v != v) m = v;
+  *out = m;
+  return FLUTAS_OK;
+}
+
+int solve_refined(flutas_psolve *h, const double *fsrc, double *p, hipStream_t s) {
+  const Geo q = halo_geo(h);
+  const long long nhalo = (long long)(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+  double *pin = p + q.interior;
+  const double normf = h->normfft;
+  // in place: keep a copy of the right-hand side for the residuals
+  const double *F = fsrc;
+  if (fsrc == p) {
+    FL_HIP(hipMemcpyAsync(h->F, p, sizeof(double) * nhalo, hipMemcpyDeviceToDevice, s));
+    F = h->F;
+  }
+  const double *fin = F + q.interior;
+  int np = 0;
+  FL_TRY(launch_absmax(F, h->nx, h->ny, h->nz, h->part, &np, s));
+  double fmax = 0.0;
+  FL_TRY(reduce_max(h, np, s, &fmax));
+  if (!(fmax == fmax) || std::isinf(fmax)) return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+  // initial FP32 solve
+  FL_TRY((launch_lines<float, double, float>(h->tx, h->kinds[0], fin, h->W32, geom_x_from_halo(h), 1.0f, s)));
+  FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[2], h->W32, h->W32, geom_y(h), 1.0f, s)));
+  FL_TRY(zsolve<float>(h, h->W32, s));
+  FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[3], h->W32, h->W32, geom_y(h), 1.0f, s)));
+  FL_TRY((launch_lines<float, float, double>(h->tx, h->kinds[1], h->W32, pin, geom_x_to_halo(h), (float)normf, s)));
+  FL_TRY(mark(h, "fp32_solve", s));
+  const OperatorDesc op = op_desc(h);
+  int it = 0;
+  double rel = 0.0;
+  for (;;) {
+    FL_TRY(launch_residual<float>(op, F, p, h->R32, h->part, &np, s));
+    FL_TRY(mark(h, "residual", s));
+    double rmax = 0.0;
+    FL_TRY(reduce_max(h, np, s, &rmax));
+    rel = (fmax > 0.0) ? rmax / fmax : rmax;
+    if (!(rel == rel)) return fail(FLUTAS_ENONFINITE, "residual is NaN");
+    if (rel <= h->tol) break;
+    if (it >= h->max_iter) {
+      h->last_iters = it;
+      h->last_relres = rel;
+      return fail(FLUTAS_ENOTCONV, "refinement did not reach the tolerance");
+    }
+    // correction solve on the residual, accumulated into p
+    FL_TRY((launch_lines<float, float, float>(h->tx, h->kinds[0], h->R32, h->W32, geom_x_dense(h), 1.0f, s)));
+    FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[2], h->W32, h->W32, geom_y(h), 1.0f, s)));
+    FL_TRY(zsolve<float>(h, h->W32, s));
+    FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[3], h->W32, h->W32, geom_y(h), 1.0f, s)));
+    FL_TRY((launch_lines<float, float, float>(h->tx, h->kinds[1], h->W32, h->R32, geom_x_dense(h), (float)normf, s)));
+    FL_TRY(launch_accumulate(h->R32, p, h->nx, h->ny, h->nz, s));
+    FL_TRY(mark(h, "correction", s));
+    ++it;
+  }
+  h->last_iters = it;
+  h->last_relres = rel;
+  return FLUTAS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *flutas_last_error(void) { return g_err.c_str(); }
+const char *flutas_version(void) { return "flutas_amd 0.1 (gfx950)"; }
+
+int flutas_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_HIP(hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+  return FLUTAS_OK;
+}
+int flutas_device_free(void *ptr) {
+  if (ptr) FL_HIP(hipFree(ptr));
+  return FLUTAS_OK;
+}
+int flutas_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  FL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FLUTAS_OK;
+}
+int flutas_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  FL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FLUTAS_OK;
+}
+int flutas_memset_d(void *dst, int value, size_t bytes) {
+  FL_HIP(hipMemset(dst, value, bytes));
+  return FLUTAS_OK;
+}
+int flutas_set_stream(void *hip_stream) {
+  g_stream = (hipStream_t)hip_stream;
+  return FLUTAS_OK;
+}
+int flutas_synchronize(void) {
+  FL_HIP(hipStreamSynchronize(g_stream));
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
+                         const char c_or_f[3], const double dli[3], int nh_d,
+                         const double *dzci, const double *dzfi, int precision,
+                         double tol, flutas_psolve **out) {
+  if (!ng || !cbc || !bc || !c_or_f || !dli || !dzci || !dzfi || !out)
+    return fail(FLUTAS_EINVAL, "null pointer");
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d) {
+    if (ng[d] < 2 || (ng[d] & 1)) return fail(FLUTAS_EINVAL, "grid sizes must be even and >= 2 (src/sanity.f90:155)");
+    if (c_or_f[d] != 'c') return fail(FLUTAS_EUNSUPPORTED, "face-centred ('f') transforms are not built");
+  }
+  for (int q = 0; q < 6; ++q)
+    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N') return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+  for (int d = 0; d < 3; ++d)
+    if ((cbc[2 * d] == 'P') != (cbc[2 * d + 1] == 'P'))
+      return fail(FLUTAS_EINVAL, "periodic BCs must be paired (src/sanity.f90:180-259)");
+  if (nh_d < 1) return fail(FLUTAS_EINVAL, "nh_d must be >= 1");
+  if (precision != FLUTAS_FP64 && precision != FLUTAS_FP32_REFINE)
+    return fail(FLUTAS_EINVAL, "precision must be 64 or 32");
+  if (precision == FLUTAS_FP32_REFINE && !(tol > 0.0)) return fail(FLUTAS_EINVAL, "tol must be > 0");
+  if (cbc[4] == 'P' && ng[2] < 4) return fail(FLUTAS_EINVAL, "periodic z needs nz >= 4");
+  FL_TRY(check_device());
+
+  auto *h = new flutas_psolve();
+  h->nx = ng[0]; h->ny = ng[1]; h->nz = ng[2];
+  std::memcpy(h->cbc, cbc, 6);
+  std::memcpy(h->bc, bc, sizeof(double) * 6);
+  std::memcpy(h->dli, dli, sizeof(double) * 3);
+  h->precision = precision;
+  h->tol = tol;
+  double n0x, n0y;
+  if (find_fft_c(cbc[0], cbc[1], h->kinds[0], h->kinds[1], n0x) ||
+      find_fft_c(cbc[2], cbc[3], h->kinds[2], h->kinds[3], n0y)) {
+    delete h;
+    return fail(FLUTAS_EUNSUPPORTED, "ND/DN pairs in x or y (REDFT11/RODFT11) are not built");
+  }
+  // normfft = 1/(norm(1)*(n+norm(2))) per direction (src/fft.f90:87,125,150)
+  h->normfft = 1.0 / ((1.0 * n0x * (h->nx + 0.0)) * n0y * (h->ny + 0.0));
+  eigen_c(h->nx, cbc[0], cbc[1], h->lamx);
+  for (double &v : h->lamx) v = v * (dli[0] * dli[0]);
+  eigen_c(h->ny, cbc[2], cbc[3], h->lamy);
+  for (double &v : h->lamy) v = v * (dli[1] * dli[1]);
+  const double *zci = dzci + (nh_d - 1), *zfi = dzfi + (nh_d - 1);
+  tridmatrix_c(cbc[4], cbc[5], h->nz, zci, zfi, h->a, h->b, h->c);
+  {
+    const double dlx[2] = {1.0 / dli[0], 1.0 / dli[0]};
+    const double dly[2] = {1.0 / dli[1], 1.0 / dli[1]};
+    bc_rhs_c(cbc + 0, bc + 0, dlx, dlx, (long)h->ny * h->nz, h->rhsbx);
+    bc_rhs_c(cbc + 2, bc + 2, dly, dly, (long)h->nx * h->nz, h->rhsby);
+    const double dlc[2] = {1.0 / zci[0], 1.0 / zci[h->nz]};
+    const double dlf[2] = {1.0 / zfi[1], 1.0 / zfi[h->nz]};
+    bc_rhs_c(cbc + 4, bc + 4, dlc, dlf, (long)h->nx * h->ny, h->rhsbz);
+  }
+  int rc = FLUTAS_OK;
+  const size_t N = (size_t)h->nx * h->ny * h->nz;
+  const size_t nhalo = (size_t)(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+  do {
+    if ((rc = h->tx.build(h->nx))) break;
+    if ((rc = h->ty.build(h->ny))) break;
+    if ((rc = dev_upload(&h->d_lamx, h->lamx))) break;
+    if ((rc = dev_upload(&h->d_lamy, h->lamy))) break;
+    if ((rc = dev_upload(&h->d_a, h->a))) break;
+    if ((rc = dev_upload(&h->d_b, h->b))) break;
+    if ((rc = dev_upload(&h->d_c, h->c))) break;
+    if ((rc = dev_upload(&h->d_rhsbx, h->rhsbx))) break;
+    if ((rc = dev_upload(&h->d_rhsby, h->rhsby))) break;
+    if ((rc = dev_upload(&h->d_rhsbz, h->rhsbz))) break;
+    hipError_t e = hipSuccess;
+    if (precision == FLUTAS_FP64) {
+      e = hipMalloc((void **)&h->W64, N * sizeof(double));
+    } else {
+      e = hipMalloc((void **)&h->W32, N * sizeof(float));
+      if (e == hipSuccess) e = hipMalloc((void **)&h->R32, N * sizeof(float));
+      if (e == hipSuccess) e = hipMalloc((void **)&h->F, nhalo * sizeof(double));
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
+    if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
+    h->npart = ((h->nx + 255) / 256) * (size_t)h->ny * h->nz;
+    if (e == hipSuccess) e = hipMalloc((void **)&h->part, h->npart * sizeof(double));
+    if (e != hipSuccess) { rc = fail(FLUTAS_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e)); break; }
+  } while (0);
+  if (rc) {
+    release(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_destroy(flutas_psolve *h) {
+  if (!h) return FLUTAS_OK;
+  (void)hipDeviceSynchronize();
+  release(h);
+  delete h;
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_coeffs(const flutas_psolve *h, double *lambdaxy, double *a,
+                         double *b, double *c, double *normfft, double *rhsbx,
+                         double *rhsby, double *rhsbz) {
+  if (!h) return fail(FLUTAS_EINVAL, "null plan");
+  if (lambdaxy)
+    for (int j = 0; j < h->ny; ++j)
+      for (int i = 0; i < h->nx; ++i) lambdaxy[i + (size_t)h->nx * j] = h->lamx[i] + h->lamy[j];
+  if (a) std::copy(h->a.begin(), h->a.end(), a);
+  if (b) std::copy(h->b.begin(), h->b.end(), b);
+  if (c) std::copy(h->c.begin(), h->c.end(), c);
+  if (normfft) *normfft = h->normfft;
+  if (rhsbx) std::copy(h->rhsbx.begin(), h->rhsbx.end(), rhsbx);
+  if (rhsby) std::copy(h->rhsby.begin(), h->rhsby.end(), rhsby);
+  if (rhsbz) std::copy(h->rhsbz.begin(), h->rhsbz.end(), rhsbz);
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_rhsb_device(const flutas_psolve *h, const double **rx,
+                              const double **ry, const double **rz) {
+  if (!h) return fail(FLUTAS_EINVAL, "null plan");
+  if (rx) *rx = h->d_rhsbx;
+  if (ry) *ry = h->d_rhsby;
+  if (rz) *rz = h->d_rhsbz;
+  return FLUTAS_OK;
+}
+
+static int solve_impl(flutas_psolve *h, const double *f, double *p) {
+  if (!h || !p || !f) return fail(FLUTAS_EINVAL, "null pointer");
+  hipStream_t s = current_stream();
+  if (h->precision == FLUTAS_FP64) return solve_fp64(h, f, p, s);
+  return solve_refined(h, f, p, s);
+}
+
+int flutas_psolve_solve_async(flutas_psolve *h, double *p) { return solve_impl(h, p, p); }
+
+int flutas_psolve_solve_into(flutas_psolve *h, const double *f, double *p) {
+  FL_TRY(solve_impl(h, f, p));
+  FL_HIP(hipStreamSynchronize(current_stream()));
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_solve(flutas_psolve *h, double *p) {
+  FL_TRY(flutas_psolve_solve_async(h, p));
+  FL_HIP(hipStreamSynchronize(current_stream()));
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_profile(flutas_psolve *h, double *p, int reps,
+                          int max_stages, double *ms, int *nstages) {
+  if (!h || !p || !ms || !nstages || reps < 1) return fail(FLUTAS_EINVAL, "bad profile arguments");
+  hipStream_t s = current_stream();
+  std::vector<double> acc;
+  h->profiling = true;
+  int rc = FLUTAS_OK;
+  for (int r = 0; r < reps && rc == FLUTAS_OK; ++r) {
+    h->nmark = 0;
+    if (h->ev.empty()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventCreate"); break; }
+      h->ev.push_back(e);
+    }
+    if (hipEventRecord(h->ev[0], s) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventRecord"); break; }
+    rc = flutas_psolve_solve_async(h, p);
+    if (rc) break;
+    if (hipEventSynchronize(h->ev[h->nmark]) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventSynchronize"); break; }
+    acc.resize(std::max<size_t>(acc.size(), h->nmark), 0.0);
+    for (int m = 0; m < h->nmark; ++m) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, h->ev[m], h->ev[m + 1]) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventElapsedTime"); break; }
+      acc[m] += t;
+    }
+  }
+  h->profiling = false;
+  if (rc) return rc;
+  const int ns = (int)std::min<size_t>(acc.size(), (size_t)max_stages);
+  for (int m = 0; m < ns; ++m) ms[m] = acc[m] / reps;
+  *nstages = ns;
+  return FLUTAS_OK;
+}
+
+const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage) {
+  if (!h || stage < 0 || stage >= (int)h->stage_names.size()) return "";
+  return h->stage_names[stage].c_str();
+}
+
+int flutas_fft_lines(int kind, int n, int howmany, long long stride,
+                     long long dist, double *data) {
+  if (!data || howmany < 1 || stride < 1) return fail(FLUTAS_EINVAL, "bad fft_lines arguments");
+  if (kind != K_R2HC && kind != K_HC2R && kind != K_REDFT01 && kind != K_REDFT10 &&
+      kind != K_RODFT01 && kind != K_RODFT10)
+    return fail(FLUTAS_EUNSUPPORTED, "r2r kind not built");
+  LineTables tab;
+  FL_TRY(tab.build(n));
+  LineGeom g;
+  g.n = n; g.L1 = howmany; g.L2 = 1;
+  g.is1 = dist; g.is2 = 0; g.ise = stride;
+  g.os1 = dist; g.os2 = 0; g.ose = stride;
+  int rc = launch_lines<double, double, double>(tab, kind, data, data, g, 1.0, current_stream());
+  hipError_t e = hipStreamSynchronize(current_stream());
+  tab.release();
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(FLUTAS_EDEVICE, hipGetErrorString(e));
+  return FLUTAS_OK;
+}
+
+int flutas_event_create(void **ev) {
+  if (!ev) return fail(FLUTAS_EINVAL, "null pointer");
+  hipEvent_t e;
+  FL_HIP(hipEventCreate(&e));
+  *ev = (void *)e;
+  return FLUTAS_OK;
+}
+int flutas_event_record(void *ev) {
+  FL_HIP(hipEventRecord((hipEvent_t)ev, current_stream()));
+  return FLUTAS_OK;
+}
+int flutas_event_elapsed_ms(void *start, void *end, float *ms) {
+  FL_HIP(hipEventSynchronize((hipEvent_t)end));
+  FL_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return FLUTAS_OK;
+}
+int flutas_event_destroy(void *ev) {
+  if (ev) FL_HIP(hipEventDestroy((hipEvent_t)ev));
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_stats(const flutas_psolve *h, int *iters, double *relres) {
+  if (!h) return fail(FLUTAS_EINVAL, "null plan");
+  if (iters) *iters = h->last_iters;
+  if (relres) *relres = h->last_relres;
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
+                           double *relres) {
+  if (!h || !f || !p || !relres) return fail(FLUTAS_EINVAL, "null pointer");
+  hipStream_t s = current_stream();
+  int np = 0;
+  FL_TRY(launch_absmax(f, h->nx, h->ny, h->nz, h->part, &np, s));
+  double fmax = 0.0, rmax = 0.0;
+  FL_TRY(reduce_max(h, np, s, &fmax));
+  FL_TRY(launch_residual<double>(op_desc(h), f, p, nullptr, h->part, &np, s));
+  FL_TRY(reduce_max(h, np, s, &rmax));
+  *relres = (fmax > 0.0) ? rmax / fmax : rmax;
+  return FLUTAS_OK;
+}
+
+// initgrid (src/initgrid.f90:17-97) with gridpoint_cluster_two_end
+// (:102-118); arrays (1-nh_d : n+nh_d), g(k) == arr[k + nh_d - 1].
+int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
+                    double *dzf, double *zc, double *zf) {
+  if (n < 1 || nh_d < 1 || !dzc || !dzf || !zc || !zf) return fail(FLUTAS_EINVAL, "bad initgrid arguments");
+  auto G = [nh_d](double *arr, int k) -> double & { return arr[k + nh_d - 1]; };
+  for (int k = 1; k <= n; ++k) {
+    const double z0 = (k - 0.0) / (1.0 * n);
+    const double z = (gr != 0.0) ? 0.5 * (1.0 + std::tanh((z0 - 0.5) * gr) / std::tanh(gr / 2.0)) : z0;
+    G(zf, k) = z * lz;
+  }
+  G(zf, 0) = 0.0;
+  for (int k = 1; k <= n; ++k) G(dzf, k) = G(zf, k) - G(zf, k - 1);
+  G(dzf, 0) = G(dzf, 1);
+  G(dzf, n + 1) = G(dzf, n);
+  for (int k = 0; k <= n; ++k) G(dzc, k) = 0.5 * (G(dzf, k) + G(dzf, k + 1));
+  G(dzc, n + 1) = G(dzc, n);
+  G(zc, 0) = -G(dzc, 0) / 2.0;
+  G(zf, 0) = 0.0;
+  for (int k = 1; k <= n + 1; ++k) {
+    G(zc, k) = G(zc, k - 1) + G(dzc, k - 1);
+    G(zf, k) = G(zf, k - 1) + G(dzf, k);
+  }
+  for (int k = 1 - nh_d; k <= 0; ++k) {
+    G(dzf, k) = G(dzf, -k + 1);
+    G(dzc, k) = G(dzc, -k);
+  }
+  for (int k = n + 1; k <= n + nh_d; ++k) {
+    G(dzf, k) = G(dzf, 2 * n - k - 1);
+    G(dzc, k) = G(dzc, 2 * n - k);
+  }
+  for (int k = 0; k >= 1 - nh_d; --k) {
+    G(zf, k) = G(zf, k + 1) - G(dzf, k);
+    G(zc, k) = G(zf, k + 1) - G(dzc, k + 1);
+  }
+  for (int k = n + 1; k <= n + nh_d; ++k) {
+    G(zf, k) = G(zf, k - 1) + G(dzf, k);
+    G(zc, k) = G(zf, k - 1) + G(dzc, k - 1);
+  }
+  return FLUTAS_OK;
+}
+
+}  // extern "C"

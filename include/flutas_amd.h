@@ -1,0 +1,198 @@
+/*
+ * flutas_amd -- MI355X (gfx950) pressure-projection step for FluTAS.
+ *
+ * C-ABI of libflutas_amd.so.  Plain pointers and sizes only.  Arrays follow
+ * the reference's Fortran conventions (column-major, i fastest) and are
+ * passed as a pointer to their FIRST element including the halo:
+ *   p, pold, rho : p(0:nx+1, 0:ny+1, 0:nz+1)           (halo 1)
+ *   u, v, w      : u(1-nh_u:nx+nh_u, ...)               (halo nh_u)
+ *   dzci, dzfi   : dzci(1-nh_d : nz+nh_d)               (halo nh_d)
+ * Field pointers ("device") are HIP device or managed allocations; the
+ * metric arrays of flutas_psolve_create are host arrays.  BC characters are
+ * 'P', 'D', 'N' in the reference's (0:1,3) order: cbc[2*dir + side].
+ *
+ * Every function returns 0 (FLUTAS_OK) or a status code; the message of the
+ * last failure on the calling thread is flutas_last_error().  Field kernels
+ * run asynchronously on the library stream (flutas_set_stream; default the
+ * null stream) unless they return host values; flutas_synchronize() waits.
+ */
+#ifndef FLUTAS_AMD_H
+#define FLUTAS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FLUTAS_OK = 0,
+  FLUTAS_ENOMEM = 12,
+  FLUTAS_EINVAL = 22,
+  FLUTAS_EDEVICE = 1001,      /* HIP runtime error, or no gfx950 device   */
+  FLUTAS_ENONFINITE = 1002,   /* NaN/Inf met in the right-hand side       */
+  FLUTAS_ENOTCONV = 1003,     /* refinement did not reach the tolerance   */
+  FLUTAS_EUNSUPPORTED = 1004  /* valid for the reference, not built here  */
+};
+
+/* Precision modes of a Poisson plan. */
+enum {
+  FLUTAS_FP64 = 64,        /* plain FP64 spectral solve                        */
+  FLUTAS_FP32_REFINE = 32  /* FP32 transforms + FP64 residual refinement       */
+};
+
+typedef struct flutas_psolve flutas_psolve;
+
+const char *flutas_last_error(void);
+const char *flutas_version(void);
+
+/* ---- device plumbing (no counterpart in the reference, which uses CUDA
+ *      managed memory for its fields: main__two_phase_inc_isot.f90:148-153) */
+int flutas_device_alloc(size_t bytes, void **ptr);
+int flutas_device_free(void *ptr);
+int flutas_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int flutas_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int flutas_memset_d(void *dst, int value, size_t bytes);
+int flutas_set_stream(void *hip_stream);
+int flutas_synchronize(void);
+
+/* ---- Poisson solver --------------------------------------------------- */
+
+/* initsolver + fftini (src/initsolver.f90:21-44, src/fft.f90:24-32).
+ * ng: global grid (one GPU = one rank); dli = 1/(dx,dy,dz);
+ * dzci, dzfi: host arrays (1-nh_d : ng[2]+nh_d).  precision: FLUTAS_FP64 or
+ * FLUTAS_FP32_REFINE; tol: relative residual target of the refined mode
+ * (ignored for FP64).  Unsupported here (EUNSUPPORTED): c_or_f 'f' and the
+ * ND/DN pairs in x or y; odd ng (EINVAL, src/sanity.f90:155-158). */
+int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
+                         const char c_or_f[3], const double dli[3], int nh_d,
+                         const double *dzci, const double *dzfi, int precision,
+                         double tol, flutas_psolve **out);
+
+/* fftend (src/fft.f90:159-179): releases the plan and its device memory. */
+int flutas_psolve_destroy(flutas_psolve *h);
+
+/* The initsolver outputs (src/initsolver.f90:34-44), host arrays:
+ * lambdaxy(nx,ny) in the halfcomplex/natural order of the CPU path,
+ * a,b,c(nz), normfft, rhsbx(ny,nz,0:1), rhsby(nx,nz,0:1), rhsbz(nx,ny,0:1).
+ * Any pointer may be NULL. */
+int flutas_psolve_coeffs(const flutas_psolve *h, double *lambdaxy, double *a,
+                         double *b, double *c, double *normfft, double *rhsbx,
+                         double *rhsby, double *rhsbz);
+
+/* Device copies of rhsbx, rhsby, rhsbz owned by the plan (for
+ * flutas_updt_rhs_b).  Pointers stay valid until destroy. */
+int flutas_psolve_rhsb_device(const flutas_psolve *h, const double **rhsbx,
+                              const double **rhsby, const double **rhsbz);
+
+/* solver_gpu, one-GPU branch (src/solver_gpu.f90:31-47, :107-229): solves
+ * the Poisson equation in place on the interior of the device array p
+ * (0:nx+1,0:ny+1,0:nz+1); halo values are neither read nor written.
+ * Returns when the result is in p. */
+int flutas_psolve_solve(flutas_psolve *h, double *p);
+
+/* Same, enqueued on the library stream; returns immediately.  The refined
+ * mode still synchronises once per refinement iteration (residual norm). */
+int flutas_psolve_solve_async(flutas_psolve *h, double *p);
+
+/* Out-of-place variant (no counterpart in the reference, whose solver works
+ * in place): reads the right-hand side from the interior of f and writes
+ * the solution to the interior of p (both halo-1 device arrays); f is left
+ * unchanged.  Same traffic as the in-place solve. */
+int flutas_psolve_solve_into(flutas_psolve *h, const double *f, double *p);
+
+/* Diagnostics of the last solve: refinement iterations and final relative
+ * residual max|f - L p| / max|f| (FP64 plans: iters = 0, relres = -1). */
+int flutas_psolve_stats(const flutas_psolve *h, int *iters, double *relres);
+
+/* Relative residual max|f - L p| / max|f| of a given pair (device arrays
+ * with halo 1), with the operator the plan inverts.  f is not modified. */
+int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
+                           double *relres);
+
+/* Per-stage timing of the solve pipeline (no counterpart in the reference,
+ * whose profiler times the whole SOLVER region, src/profiler.f90:83-205):
+ * runs `reps` solves on p and returns the mean duration in ms of each stage,
+ * measured with HIP events on the library stream.  *nstages receives the
+ * number of stages (<= max_stages); flutas_psolve_stage_name names them.
+ * p is overwritten by each solve, so pass a scratch copy of the RHS. */
+int flutas_psolve_profile(flutas_psolve *h, double *p, int reps,
+                          int max_stages, double *ms, int *nstages);
+const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
+
+/* fft (src/fft.f90:181-193): executes one r2r kind in place on `howmany`
+ * device lines of length n (element stride `stride`, line distance
+ * `dist`), FP64, unnormalised FFTW definitions.  kind uses FFTW's
+ * numbering (src/fftw.f90:38-61); built: R2HC(0), HC2R(1), REDFT01(4),
+ * REDFT10(5), RODFT01(8), RODFT10(9); n even. */
+int flutas_fft_lines(int kind, int n, int howmany, long long stride,
+                     long long dist, double *data);
+
+/* HIP events on the library stream, for host-side timing of any sequence. */
+int flutas_event_create(void **ev);
+int flutas_event_record(void *ev);
+int flutas_event_elapsed_ms(void *start, void *end, float *ms);
+int flutas_event_destroy(void *ev);
+
+/* ---- field kernels of the projection step ------------------------------ */
+
+/* fillps (src/fillps.f90:16-26), _CONSTANT_COEFFS_POISSON: p interior =
+ * rho0 * div(u,v,w) * dti.  dzfi is a device array. */
+int flutas_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzfi, double dti,
+                  double rho0, const double *u, const double *v,
+                  const double *w, double *p);
+
+/* pres_tw_src (src/source.f90:247-259), two-phase pressure splitting;
+ * const_coeff = 1 is the _CONSTANT_COEFFS_POISSON branch. */
+int flutas_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi,
+                       double dzi, int nh_d, int nh_u, const double *dzci,
+                       double rho0i, double f_t12, double f_t12_o,
+                       const double *p, const double *pold, const double *rho,
+                       double *u, double *v, double *w, int const_coeff);
+
+/* pres_sp_src (src/source.f90:311-323), single-phase. */
+int flutas_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi,
+                       double dyi, double dzi, int nh_d, int nh_u,
+                       const double *dzci, double rho0i, const double *pold,
+                       double *u, double *v, double *w);
+
+/* updt_rhs_b (src/bound.f90:829-944) at one rank; rhsb* device arrays. */
+int flutas_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
+                      const char cbc[6], const double *rhsbx,
+                      const double *rhsby, const double *rhsbz, double *p);
+
+/* boundp (src/bound.f90:146-225) at one rank, nh_p = 1.  dl = (dx,dy,dz);
+ * dzc_0 = dzc(0), dzc_n = dzc(nz) (the only z metrics it reads). */
+int flutas_boundp(const char cbc[6], const int n[3], const double bc[6],
+                  const double dl[3], double dzc_0, double dzc_n, double *p);
+
+/* correc (src/correc.f90:16-29); const_coeff = 1 divides by rho0, else by
+ * the arithmetic face density of rho (rho may be NULL when const_coeff). */
+int flutas_correc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzci, double dt,
+                  double rho0, const double *p, double *u, double *v,
+                  double *w, const double *rho, int const_coeff);
+
+/* chkdiv (src/chkdiv.f90:18-31): returns the total and the maximum absolute
+ * divergence (synchronous). */
+int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
+                  int nh_d, int nh_u, const double *dzfi, const double *u,
+                  const double *v, const double *w, double *divtot,
+                  double *divmax);
+
+/* pold = p on the interior (main__two_phase_inc_isot.f90:656-673). */
+int flutas_copy_interior(int nx, int ny, int nz, const double *src, double *dst);
+
+/* p = pold + p on the interior (main__two_phase_inc_isot.f90:698-714). */
+int flutas_add_interior(int nx, int ny, int nz, const double *pold, double *p);
+
+/* initgrid with two-end tanh clustering (src/initgrid.f90:17-97,
+ * :102-118); host arrays (1-nh_d : n+nh_d). */
+int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
+                    double *dzf, double *zc, double *zf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -25,6 +25,13 @@
 
 typedef double complex cplx;
 
+#ifdef _OPENMP
+#include <omp.h>
+int or_max_threads(void) { return omp_get_max_threads(); }
+#else
+int or_max_threads(void) { return 1; }
+#endif
+
 /* ------------------------------------------------------------------------ */
 /* Complex DFT of any length: recursive mixed-radix decimation in time.      */
 /* ------------------------------------------------------------------------ */

@@ -27,6 +27,9 @@ enum {
   OR_RODFT00 = 7, OR_RODFT01 = 8, OR_RODFT10 = 9, OR_RODFT11 = 10
 };
 
+/* OpenMP threads the batched loops use (1 without OpenMP). */
+int or_max_threads(void);
+
 /* --- transforms ---------------------------------------------------------- */
 /* Unnormalised FFTW r2r definitions, evaluated as direct O(n^2) sums in long
  * double.  The definition the fast path is checked against. */

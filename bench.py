@@ -113,9 +113,7 @@ def main():
     ms, wall_ms = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
     relres64 = plan.residual(f, p)
     log(f"bench: fp64 {ms:.3f} ms/solve (wall {wall_ms:.3f}), relres {relres64:.2e}")
-    scratch = api.DeviceArray(f.shape)
-    api.copy_interior(nx, ny, nz, f, scratch)
-    stages = plan.profile(scratch, args.profile_reps)
+    stages = plan.profile(f, p, args.profile_reps)
     stage_rows = []
     for name, t in stages:
         b = STAGE_BYTES_FP64.get(name, 16) * npts
@@ -125,7 +123,6 @@ def main():
                                      for r in stage_rows))
     solve_gbps = SOLVE_BYTES_FP64 * npts / (ms * 1e6)
     plan.destroy()
-    del scratch
 
     # ---- FP32 + FP64 refinement ---------------------------------------
     refine = None
@@ -136,9 +133,12 @@ def main():
                              args.warmup, api)
         iters, rel = plan32.stats()
         check = plan32.residual(f, p)
+        stages32 = plan32.profile(f, p, max(1, args.profile_reps // 2))
+        log("bench: refine stages " + ", ".join(f"{s} {t:.3f}" for s, t in stages32))
         refine = {"ns_per_gridpoint": ms32 * 1e6 / npts, "ms_per_solve": ms32,
                   "iterations": iters, "relres": rel, "relres_check": check,
                   "throughput_vs_fp64": ms / ms32,
+                  "stages_ms": [[s, t] for s, t in stages32],
                   "algorithmic_bytes_per_point": 48 + 96 * iters + 24}
         log(f"bench: fp32+refine {ms32:.3f} ms/solve, {iters} iters, relres {rel:.2e}")
         plan32.destroy()

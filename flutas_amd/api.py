@@ -213,10 +213,11 @@ class PoissonPlan:
         _check(lib.flutas_psolve_residual(_VP(self._h), f.c, p.c, C.byref(rr)))
         return rr.value
 
-    def profile(self, p, reps):
+    def profile(self, f, p, reps):
+        """Mean per-stage durations (ms) of `reps` solves f -> p."""
         ms = (C.c_double * 32)()
         ns = C.c_int()
-        _check(lib.flutas_psolve_profile(_VP(self._h), p.c, C.c_int(reps),
+        _check(lib.flutas_psolve_profile(_VP(self._h), f.c, p.c, C.c_int(reps),
                                          C.c_int(32), ms, C.byref(ns)))
         return [(lib.flutas_psolve_stage_name(_VP(self._h), s).decode(), ms[s])
                 for s in range(ns.value)]

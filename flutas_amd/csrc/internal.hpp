@@ -93,6 +93,28 @@ int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
                            const double *b, const double *c, int nx, int ny,
                            int nz, int pin, hipStream_t s);
 
+// ---- fused four-pass pipeline (non-periodic z) ---------------------------
+// Work layout Wt[i][k][j]; see fused.hip.
+bool fused_supported(int n);
+int fused_group_planes(int ny, int elem_bytes);
+template <typename T, typename Tin>
+int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
+                 long long is2, T *Wt, int ny, int nz, hipStream_t s);
+template <typename T, typename Tout>
+int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
+                 long long os1, long long os2, int ny, int nz, T scale,
+                 int accumulate, hipStream_t s);
+template <typename T>
+int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, double *ck, int nx,
+                  int nz, const double *lamx, const double *lamy,
+                  const double *a, const double *b, const double *c, int pin,
+                  hipStream_t s);
+template <typename T>
+int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
+                  int nx, int nz, const double *lamx, const double *lamy,
+                  const double *a, const double *b, const double *c,
+                  hipStream_t s);
+
 // ---- refinement helpers ----------------------------------------------------
 struct OperatorDesc {
   int nx, ny, nz;

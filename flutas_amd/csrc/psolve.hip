@@ -17,7 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <mutex>
+#include <cstdlib>
 
 namespace flutas {
 
@@ -124,6 +124,8 @@ struct flutas_psolve {
   float *R32 = nullptr;    // residual (refined mode)
   double *part = nullptr;  // reduction partials
   size_t npart = 0;
+  bool fused = false;      // four-pass pipeline (fused.hip) in use
+  double *CK = nullptr;    // z elimination checkpoints of the fused pipeline
   int last_iters = 0;
   double last_relres = -1.0;
   // stage timing (flutas_psolve_profile)
@@ -148,7 +150,7 @@ void release(flutas_psolve *h) {
                   (void *)h->d_b, (void *)h->d_c, (void *)h->d_rhsbx,
                   (void *)h->d_rhsby, (void *)h->d_rhsbz, (void *)h->W64,
                   (void *)h->W32, (void *)h->D, (void *)h->P2, (void *)h->F,
-                  (void *)h->R32, (void *)h->part})
+                  (void *)h->R32, (void *)h->part, (void *)h->CK})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
@@ -231,20 +233,59 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
                           h->nx, h->ny, h->nz, pin, s);
 }
 
+// One spectral solve in storage/arithmetic T: the right-hand side `in`
+// (halo-1 layout at its interior origin when in_halo, else dense nx*ny*nz)
+// -> out (interior origin of a halo-1 FP64 field): out = S(in), or
+// out += S(in) when accumulate.
+template <typename T, typename Tin>
+int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
+             int accumulate, T *W, hipStream_t s) {
+  const Geo q = halo_geo(h);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const long long is1 = in_halo ? q.hx : nx;
+  const long long is2 = in_halo ? q.hxy : (long long)nx * ny;
+  if (h->fused) {
+    const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N');
+    FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ny, nz, s)));
+    FL_TRY(mark(h, "P1_x_fwd", s));
+    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
+                            h->d_a, h->d_b, h->d_c, pin, s));
+    FL_TRY(mark(h, "P2_yz_fwd", s));
+    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
+                            h->d_a, h->d_b, h->d_c, s));
+    FL_TRY(mark(h, "P3_zy_bwd", s));
+    FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, out, q.hx, q.hxy, ny, nz,
+                                    (T)h->normfft, accumulate, s)));
+    FL_TRY(mark(h, "P4_x_bwd", s));
+    return FLUTAS_OK;
+  }
+  LineGeom gx = geom_x_dense(h);
+  gx.is1 = is1;
+  gx.is2 = is2;
+  FL_TRY((launch_lines<T, Tin, T>(h->tx, h->kinds[0], in, W, gx, T(1), s)));
+  FL_TRY(mark(h, "x_fwd", s));
+  FL_TRY((launch_lines<T, T, T>(h->ty, h->kinds[2], W, W, geom_y(h), T(1), s)));
+  FL_TRY(mark(h, "y_fwd", s));
+  FL_TRY(zsolve<T>(h, W, s));
+  FL_TRY(mark(h, "z_solve", s));
+  FL_TRY((launch_lines<T, T, T>(h->ty, h->kinds[3], W, W, geom_y(h), T(1), s)));
+  FL_TRY(mark(h, "y_bwd", s));
+  if (!accumulate) {
+    FL_TRY((launch_lines<T, T, double>(h->tx, h->kinds[1], W, out, geom_x_to_halo(h),
+                                       (T)h->normfft, s)));
+  } else {
+    // only the refined mode accumulates: through the dense FP32 residual buffer
+    FL_TRY((launch_lines<T, T, T>(h->tx, h->kinds[1], W, (T *)h->R32, geom_x_dense(h),
+                                  (T)h->normfft, s)));
+    FL_TRY(launch_accumulate(h->R32, out - q.interior, nx, ny, nz, s));
+  }
+  FL_TRY(mark(h, "x_bwd", s));
+  return FLUTAS_OK;
+}
+
 int solve_fp64(flutas_psolve *h, const double *f, double *p, hipStream_t s) {
   const Geo q = halo_geo(h);
-  double *pin = p + q.interior;
-  const double *fin = f + q.interior;
-  FL_TRY((launch_lines<double, double, double>(h->tx, h->kinds[0], fin, h->W64, geom_x_from_halo(h), 1.0, s)));
-  FL_TRY(mark(h, "x_fwd", s));
-  FL_TRY((launch_lines<double, double, double>(h->ty, h->kinds[2], h->W64, h->W64, geom_y(h), 1.0, s)));
-  FL_TRY(mark(h, "y_fwd", s));
-  FL_TRY(zsolve<double>(h, h->W64, s));
-  FL_TRY(mark(h, "z_solve", s));
-  FL_TRY((launch_lines<double, double, double>(h->ty, h->kinds[3], h->W64, h->W64, geom_y(h), 1.0, s)));
-  FL_TRY(mark(h, "y_bwd", s));
-  FL_TRY((launch_lines<double, double, double>(h->tx, h->kinds[1], h->W64, pin, geom_x_to_halo(h), h->normfft, s)));
-  FL_TRY(mark(h, "x_bwd", s));
+  FL_TRY((spectral<double, double>(h, f + q.interior, true, p + q.interior, 0, h->W64, s)));
   h->last_iters = 0;
   h->last_relres = -1.0;
   return FLUTAS_OK;
@@ -289,12 +330,8 @@ int solve_refined(flutas_psolve *h, const double *fsrc, double *p, hipStream_t s
   FL_TRY(reduce_max(h, np, s, &fmax));
   if (!(fmax == fmax) || std::isinf(fmax)) return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
   // initial FP32 solve
-  FL_TRY((launch_lines<float, double, float>(h->tx, h->kinds[0], fin, h->W32, geom_x_from_halo(h), 1.0f, s)));
-  FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[2], h->W32, h->W32, geom_y(h), 1.0f, s)));
-  FL_TRY(zsolve<float>(h, h->W32, s));
-  FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[3], h->W32, h->W32, geom_y(h), 1.0f, s)));
-  FL_TRY((launch_lines<float, float, double>(h->tx, h->kinds[1], h->W32, pin, geom_x_to_halo(h), (float)normf, s)));
-  FL_TRY(mark(h, "fp32_solve", s));
+  (void)normf;
+  FL_TRY((spectral<float, double>(h, fin, true, pin, 0, h->W32, s)));
   const OperatorDesc op = op_desc(h);
   int it = 0;
   double rel = 0.0;
@@ -312,13 +349,7 @@ int solve_refined(flutas_psolve *h, const double *fsrc, double *p, hipStream_t s
       return fail(FLUTAS_ENOTCONV, "refinement did not reach the tolerance");
     }
     // correction solve on the residual, accumulated into p
-    FL_TRY((launch_lines<float, float, float>(h->tx, h->kinds[0], h->R32, h->W32, geom_x_dense(h), 1.0f, s)));
-    FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[2], h->W32, h->W32, geom_y(h), 1.0f, s)));
-    FL_TRY(zsolve<float>(h, h->W32, s));
-    FL_TRY((launch_lines<float, float, float>(h->ty, h->kinds[3], h->W32, h->W32, geom_y(h), 1.0f, s)));
-    FL_TRY((launch_lines<float, float, float>(h->tx, h->kinds[1], h->W32, h->R32, geom_x_dense(h), (float)normf, s)));
-    FL_TRY(launch_accumulate(h->R32, p, h->nx, h->ny, h->nz, s));
-    FL_TRY(mark(h, "correction", s));
+    FL_TRY((spectral<float, float>(h, h->R32, false, pin, 1, h->W32, s)));
     ++it;
   }
   h->last_iters = it;
@@ -438,8 +469,18 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
       if (e == hipSuccess) e = hipMalloc((void **)&h->R32, N * sizeof(float));
       if (e == hipSuccess) e = hipMalloc((void **)&h->F, nhalo * sizeof(double));
     }
-    if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
-    if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
+    // the fused pipeline covers non-periodic z and the compiled line lengths
+    const char *env = std::getenv("FLUTAS_UNFUSED");
+    h->fused = cbc[4] != 'P' && fused_supported(h->nx) && fused_supported(h->ny) &&
+               !(env && env[0] == '1');
+    if (h->fused) {
+      const int gp = fused_group_planes(h->ny, precision == FLUTAS_FP64 ? 8 : 4);
+      const size_t ngrp = (h->nz + gp - 1) / gp;
+      if (e == hipSuccess) e = hipMalloc((void **)&h->CK, (size_t)h->nx * ngrp * h->ny * sizeof(double));
+    } else {
+      if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
+      if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
+    }
     h->npart = ((h->nx + 255) / 256) * (size_t)h->ny * h->nz;
     if (e == hipSuccess) e = hipMalloc((void **)&h->part, h->npart * sizeof(double));
     if (e != hipSuccess) { rc = fail(FLUTAS_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e)); break; }
@@ -508,9 +549,9 @@ int flutas_psolve_solve(flutas_psolve *h, double *p) {
   return FLUTAS_OK;
 }
 
-int flutas_psolve_profile(flutas_psolve *h, double *p, int reps,
-                          int max_stages, double *ms, int *nstages) {
-  if (!h || !p || !ms || !nstages || reps < 1) return fail(FLUTAS_EINVAL, "bad profile arguments");
+int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
+                          int reps, int max_stages, double *ms, int *nstages) {
+  if (!h || !f || !p || !ms || !nstages || reps < 1) return fail(FLUTAS_EINVAL, "bad profile arguments");
   hipStream_t s = current_stream();
   std::vector<double> acc;
   h->profiling = true;
@@ -523,7 +564,7 @@ int flutas_psolve_profile(flutas_psolve *h, double *p, int reps,
       h->ev.push_back(e);
     }
     if (hipEventRecord(h->ev[0], s) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventRecord"); break; }
-    rc = flutas_psolve_solve_async(h, p);
+    rc = solve_impl(h, f, p);
     if (rc) break;
     if (hipEventSynchronize(h->ev[h->nmark]) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventSynchronize"); break; }
     acc.resize(std::max<size_t>(acc.size(), h->nmark), 0.0);

@@ -1,0 +1,490 @@
+// Compile-time-sized FFT building blocks for tiles of lines held in LDS.
+//
+// A tile is L lines of n = 2N reals (line stride LS reals) in LDS.  The whole
+// workgroup (NT threads) runs a Stockham FFT of length N on every line: each
+// pass reads the butterfly inputs of all its butterflies into registers,
+// synchronises, and writes the outputs in place.  The real split of
+// R2HC/HC2R and the cosine/sine phases of REDFT10/01, RODFT10/01 are
+// separate tile passes with the same read/synchronise/write structure, so a
+// line is read from HBM once and written once per kernel.
+#pragma once
+
+#include "internal.hpp"
+
+namespace flutas {
+namespace tile {
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> sub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> mul(cx<T> a, cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cj(cx<T> a) { return {a.x, -a.y}; }
+template <typename T, int S>
+__device__ __forceinline__ cx<T> msi(cx<T> a) {  // multiply by S*i
+  return (S > 0) ? cx<T>{-a.y, a.x} : cx<T>{a.y, -a.x};
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void d2(cx<T> *v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = add(a, b);
+  v[1] = sub(a, b);
+}
+template <typename T, int S>
+__device__ __forceinline__ void d3(cx<T> *v) {
+  const T h = T(0.866025403784438646763723170752936183L);
+  cx<T> t1 = add(v[1], v[2]);
+  cx<T> t2 = {v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y};
+  cx<T> d = sub(v[1], v[2]);
+  cx<T> t3 = msi<T, S>(cx<T>{h * d.x, h * d.y});
+  v[0] = add(v[0], t1);
+  v[1] = add(t2, t3);
+  v[2] = sub(t2, t3);
+}
+template <typename T, int S>
+__device__ __forceinline__ void d4(cx<T> *v) {
+  cx<T> t0 = add(v[0], v[2]), t1 = sub(v[0], v[2]);
+  cx<T> t2 = add(v[1], v[3]), t3 = msi<T, S>(sub(v[1], v[3]));
+  v[0] = add(t0, t2);
+  v[1] = add(t1, t3);
+  v[2] = sub(t0, t2);
+  v[3] = sub(t1, t3);
+}
+template <typename T, int S>
+__device__ __forceinline__ void d8(cx<T> *v) {
+  const T r = T(0.707106781186547524400844362104849039L);
+  cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  d4<T, S>(e);
+  d4<T, S>(o);
+  cx<T> o1 = {r * (o[1].x - S * o[1].y), r * (o[1].y + S * o[1].x)};
+  cx<T> o2 = msi<T, S>(o[2]);
+  cx<T> o3 = {r * (-o[3].x - S * o[3].y), r * (-o[3].y + S * o[3].x)};
+  v[0] = add(e[0], o[0]);
+  v[4] = sub(e[0], o[0]);
+  v[1] = add(e[1], o1);
+  v[5] = sub(e[1], o1);
+  v[2] = add(e[2], o2);
+  v[6] = sub(e[2], o2);
+  v[3] = add(e[3], o3);
+  v[7] = sub(e[3], o3);
+}
+// 16-point DFT as 4 x 4 with the internal twiddles w16^(n2*k1)
+template <typename T, int S>
+__device__ __forceinline__ void d16(cx<T> *v) {
+  const T c1 = T(0.923879532511286756128183189396788933L);  // cos(pi/8)
+  const T s1 = T(0.382683432365089771728459984030398867L);  // sin(pi/8)
+  const T r = T(0.707106781186547524400844362104849039L);
+  cx<T> A[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cx<T> q[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+    d4<T, S>(q);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) A[n2][k1] = q[k1];
+  }
+  // twiddles w^m, w = exp(S 2 pi i / 16), m = n2*k1
+  auto tw = [&](cx<T> a, int m) -> cx<T> {
+    switch (m) {
+      case 0: return a;
+      case 1: return mul(a, cx<T>{c1, S * s1});
+      case 2: return mul(a, cx<T>{r, S * r});
+      case 3: return mul(a, cx<T>{s1, S * c1});
+      case 4: return msi<T, S>(a);
+      case 6: return mul(a, cx<T>{-r, S * r});
+      case 9: return mul(a, cx<T>{-c1, -S * s1});
+      default: return a;
+    }
+  };
+#pragma unroll
+  for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) A[n2][k1] = tw(A[n2][k1], n2 * k1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cx<T> q[4] = {A[0][k1], A[1][k1], A[2][k1], A[3][k1]};
+    d4<T, S>(q);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = q[k2];
+  }
+}
+
+template <typename T, int S, int R>
+__device__ __forceinline__ void dft(cx<T> *v) {
+  if constexpr (R == 2) d2<T, S>(v);
+  else if constexpr (R == 3) d3<T, S>(v);
+  else if constexpr (R == 4) d4<T, S>(v);
+  else if constexpr (R == 8) d8<T, S>(v);
+  else if constexpr (R == 16) d16<T, S>(v);
+  else static_assert(R == 2, "radix not built");
+}
+
+// Compile-time radix plans of the complex length N = n/2.
+template <int N>
+struct Plan;
+#define FL_PLAN(NN, A, B, C, NS)                \
+  template <>                                   \
+  struct Plan<NN> {                             \
+    static constexpr int nst = NS;              \
+    static constexpr int r[3] = {A, B, C};      \
+  };
+FL_PLAN(16, 16, 1, 1, 1)
+FL_PLAN(24, 8, 3, 1, 2)
+FL_PLAN(32, 8, 4, 1, 2)
+FL_PLAN(48, 16, 3, 1, 2)
+FL_PLAN(64, 16, 4, 1, 2)
+FL_PLAN(96, 8, 4, 3, 3)
+FL_PLAN(128, 16, 8, 1, 2)
+FL_PLAN(192, 16, 4, 3, 3)
+FL_PLAN(256, 16, 16, 1, 2)
+FL_PLAN(384, 16, 8, 3, 3)
+FL_PLAN(512, 8, 8, 8, 3)
+FL_PLAN(1024, 16, 16, 4, 3)
+#undef FL_PLAN
+
+// One Stockham pass of radix R over L lines (complex line stride LSC).
+template <typename T, int S, int N, int R, int NS, int L, int NT, int LSC>
+__device__ __forceinline__ void pass(cx<T> *buf, const T *__restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TOT = L * NB;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  cx<T> v[PER][R];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    if (TOT % NT == 0 || t < TOT) {
+      const int l = t / NB, b = t - (t / NB) * NB;
+      const int j = b % NS;
+      const cx<T> *x = buf + l * LSC;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cx<T> a = x[b + r * NB];
+        if (NS > 1 && r > 0) {
+          const int m = r * j * (N / (NS * R));
+          cx<T> w = {tw[2 * m], tw[2 * m + 1]};
+          if (S > 0) w.y = -w.y;
+          a = mul(a, w);
+        }
+        v[u][r] = a;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    if (TOT % NT == 0 || t < TOT) {
+      const int l = t / NB, b = t - (t / NB) * NB;
+      const int j = b % NS;
+      dft<T, S, R>(v[u]);
+      cx<T> *y = buf + l * LSC;
+      const int base = (b / NS) * NS * R + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[base + r * NS] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Complex FFT of length N on L lines, sign S; tw = exp(-2 pi i m / N).
+template <typename T, int S, int N, int L, int NT, int LSC>
+__device__ __forceinline__ void cfft(cx<T> *buf, const T *__restrict__ tw) {
+  using P = Plan<N>;
+  pass<T, S, N, P::r[0], 1, L, NT, LSC>(buf, tw);
+  if constexpr (P::nst > 1) pass<T, S, N, P::r[1], P::r[0], L, NT, LSC>(buf, tw);
+  if constexpr (P::nst > 2)
+    pass<T, S, N, P::r[2], P::r[0] * P::r[1], L, NT, LSC>(buf, tw);
+}
+
+__device__ __forceinline__ bool is_fwd(int kind) {
+  return kind == K_R2HC || kind == K_REDFT10 || kind == K_RODFT10;
+}
+__device__ __forceinline__ bool is_trig(int kind) {
+  return kind != K_R2HC && kind != K_HC2R;
+}
+__device__ __forceinline__ bool is_sine(int kind) {
+  return kind == K_RODFT10 || kind == K_RODFT01;
+}
+// Makhoul's reordering position of element e (v_m = x_2m, v_{n-1-m} = x_2m+1)
+__device__ __forceinline__ int mk_pos(int e, int n) {
+  return (e & 1) ? (n - 1 - (e >> 1)) : (e >> 1);
+}
+
+// Forward real split and kind phases, in place on L lines of n = 2N reals:
+// the complex FFT Z of the packed line becomes the R2HC halfcomplex output
+// or the REDFT10 / RODFT10 output in natural order.
+template <typename T, int N, int L, int NT, int LS>
+__device__ __forceinline__ void post_fwd(T *sm, int kind, const T *__restrict__ twh,
+                                         const T *__restrict__ twq) {
+  constexpr int n = 2 * N;
+  constexpr int NP = N / 2 + 1;  // pairs (k, N-k), k = 0..N/2
+  constexpr int TOT = L * NP;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  const bool trig = is_trig(kind), sine = is_sine(kind);
+  T o[PER][4];
+  int pos[PER][4];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    pos[u][0] = pos[u][1] = pos[u][2] = pos[u][3] = -1;
+    if (t < TOT) {
+      const int l = t / NP, k = t - (t / NP) * NP;
+      const T *line = sm + l * LS;
+      const cx<T> *Z = reinterpret_cast<const cx<T> *>(line);
+      // the two members of the pair: k and N-k (Z_N := Z_0)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kk = (h == 0) ? k : N - k;
+        if (h == 1 && (k == 0 || 2 * k == N)) {
+          if (k == 0) {
+            // X_N = Re Z0 - Im Z0 (real), X_0 handled by h == 0
+            const cx<T> z0 = Z[0];
+            const T XN = z0.x - z0.y;
+            if (!trig) { o[u][2] = XN; pos[u][2] = N; }
+            else {
+              const T c = twq[2 * N];  // cos(pi/4)
+              const T y = T(2) * (XN * c);
+              o[u][2] = y;
+              pos[u][2] = sine ? (n - 1 - N) : N;
+            }
+          }
+          continue;
+        }
+        const cx<T> Zk = Z[kk == N ? 0 : kk];
+        const cx<T> Zm = Z[kk == 0 ? 0 : N - kk];
+        const cx<T> A = add(Zk, cj(Zm)), B = sub(Zk, cj(Zm));
+        const cx<T> wB = mul(cx<T>{twh[2 * kk], twh[2 * kk + 1]}, B);
+        const cx<T> X = {T(0.5) * A.x + T(0.5) * wB.y, T(0.5) * A.y - T(0.5) * wB.x};
+        int p0, p1;
+        T v0, v1;
+        if (!trig) {
+          v0 = X.x; p0 = kk;
+          v1 = X.y; p1 = (kk > 0 && kk < N) ? n - kk : -1;
+        } else {
+          const T c = twq[2 * kk], sn = -twq[2 * kk + 1];
+          v0 = T(2) * (X.x * c + X.y * sn);
+          v1 = T(2) * (X.x * sn - X.y * c);
+          p0 = sine ? n - 1 - kk : kk;
+          p1 = (kk > 0 && kk < N) ? (sine ? kk - 1 : n - kk) : -1;
+        }
+        o[u][2 * h] = v0; pos[u][2 * h] = p0;
+        o[u][2 * h + 1] = v1; pos[u][2 * h + 1] = p1;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    if (t < TOT) {
+      T *line = sm + (t / NP) * LS;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (pos[u][q] >= 0) line[pos[u][q]] = o[u][q];
+    }
+  }
+  __syncthreads();
+}
+
+// halfcomplex element k (0 <= k <= N) of a real line
+template <typename T, int N>
+__device__ __forceinline__ cx<T> hc(const T *X, int k) {
+  if (k == 0 || k == N) return {X[k], T(0)};
+  return {X[k], X[2 * N - k]};
+}
+// V'_k = exp(i pi k / 2n)(X_k - i X_{n-k}), X_n := 0; `rev` reads X_{n-1-k}
+template <typename T, int N>
+__device__ __forceinline__ cx<T> vprime(const T *X, int k, const T *__restrict__ twq,
+                                        bool rev) {
+  constexpr int n = 2 * N;
+  const T xk = rev ? X[n - 1 - k] : X[k];
+  const T xnk = (k == 0) ? T(0) : (rev ? X[k - 1] : X[n - k]);
+  const cx<T> q = {twq[2 * k], -twq[2 * k + 1]};
+  return mul(q, cx<T>{xk, -xnk});
+}
+
+// Inverse pre-processing in place: real input line X (halfcomplex for
+// HC2R, natural for REDFT01/RODFT01) becomes the complex array
+// Z'_k = (X_k + conj X_{N-k}) + i e^{2 pi i k/n}(X_k - conj X_{N-k}).
+template <typename T, int N, int L, int NT, int LS>
+__device__ __forceinline__ void pre_inv(T *sm, int kind, const T *__restrict__ twh,
+                                        const T *__restrict__ twq) {
+  constexpr int NP = N / 2 + 1;
+  constexpr int TOT = L * NP;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  const bool trig = is_trig(kind), sine = is_sine(kind);
+  cx<T> z[PER][2];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    if (t < TOT) {
+      const int l = t / NP, k = t - (t / NP) * NP;
+      const T *X = sm + l * LS;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int kk = (h == 0) ? k : N - k;
+        if (h == 1 && (k == 0 || 2 * k == N)) continue;
+        cx<T> Xk, Xm;
+        if (!trig) {
+          Xk = hc<T, N>(X, kk);
+          Xm = hc<T, N>(X, N - kk);
+        } else {
+          Xk = vprime<T, N>(X, kk, twq, sine);
+          Xm = vprime<T, N>(X, N - kk, twq, sine);
+        }
+        const cx<T> A = add(Xk, cj(Xm)), B = sub(Xk, cj(Xm));
+        const cx<T> w = {twh[2 * kk], -twh[2 * kk + 1]};
+        z[u][h] = add(A, msi<T, 1>(mul(w, B)));
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = tid + u * NT;
+    if (t < TOT) {
+      const int l = t / NP, k = t - (t / NP) * NP;
+      cx<T> *Z = reinterpret_cast<cx<T> *>(sm + l * LS);
+      Z[k] = z[u][0];
+      if (k != 0 && 2 * k != N) Z[N - k] = z[u][1];
+    }
+  }
+  __syncthreads();
+}
+
+// ---- ping-pong variants: read buffer `src`, write buffer `dst`, one
+// barrier at the end; no register staging (lower register pressure at the
+// cost of a second LDS tile).
+
+template <typename T, int S, int N, int R, int NS, int L, int NT, int LSC>
+__device__ __forceinline__ void pass_pp(const cx<T> *src, cx<T> *dst,
+                                        const T *__restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TOT = L * NB;
+  for (int t = threadIdx.x; t < TOT; t += NT) {
+    const int l = t / NB, b = t - (t / NB) * NB;
+    const int j = b % NS;
+    const cx<T> *x = src + l * LSC;
+    cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cx<T> a = x[b + r * NB];
+      if (NS > 1 && r > 0) {
+        const int m = r * j * (N / (NS * R));
+        cx<T> w = {tw[2 * m], tw[2 * m + 1]};
+        if (S > 0) w.y = -w.y;
+        a = mul(a, w);
+      }
+      v[r] = a;
+    }
+    dft<T, S, R>(v);
+    cx<T> *y = dst + l * LSC;
+    const int base = (b / NS) * NS * R + j;
+#pragma unroll
+    for (int r = 0; r < R; ++r) y[base + r * NS] = v[r];
+  }
+  __syncthreads();
+}
+
+// Complex FFT with ping-pong buffers: the result ends in `b` after an odd
+// number of passes (Plan<N>::nst), back in `a` after an even number.
+template <typename T, int S, int N, int L, int NT, int LSC>
+__device__ __forceinline__ void cfft_pp(cx<T> *a, cx<T> *b, const T *__restrict__ tw) {
+  using P = Plan<N>;
+  pass_pp<T, S, N, P::r[0], 1, L, NT, LSC>(a, b, tw);
+  if constexpr (P::nst > 1) pass_pp<T, S, N, P::r[1], P::r[0], L, NT, LSC>(b, a, tw);
+  if constexpr (P::nst > 2)
+    pass_pp<T, S, N, P::r[2], P::r[0] * P::r[1], L, NT, LSC>(a, b, tw);
+}
+
+// post_fwd from src (complex FFT of the packed lines) into dst.
+template <typename T, int N, int L, int NT, int LS>
+__device__ __forceinline__ void post_fwd_pp(const T *src, T *dst, int kind,
+                                            const T *__restrict__ twh,
+                                            const T *__restrict__ twq) {
+  constexpr int n = 2 * N;
+  constexpr int NP = N / 2 + 1;
+  constexpr int TOT = L * NP;
+  const bool trig = is_trig(kind), sine = is_sine(kind);
+  for (int t = threadIdx.x; t < TOT; t += NT) {
+    const int l = t / NP, k = t - (t / NP) * NP;
+    const cx<T> *Z = reinterpret_cast<const cx<T> *>(src + l * LS);
+    T *Y = dst + l * LS;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kk = (h == 0) ? k : N - k;
+      if (h == 1 && (k == 0 || 2 * k == N)) {
+        if (k == 0) {
+          const cx<T> z0 = Z[0];
+          const T XN = z0.x - z0.y;
+          if (!trig) Y[N] = XN;
+          else Y[sine ? (n - 1 - N) : N] = T(2) * (XN * twq[2 * N]);
+        }
+        continue;
+      }
+      const cx<T> Zk = Z[kk == N ? 0 : kk];
+      const cx<T> Zm = Z[kk == 0 ? 0 : N - kk];
+      const cx<T> A = add(Zk, cj(Zm)), B = sub(Zk, cj(Zm));
+      const cx<T> wB = mul(cx<T>{twh[2 * kk], twh[2 * kk + 1]}, B);
+      const cx<T> X = {T(0.5) * A.x + T(0.5) * wB.y, T(0.5) * A.y - T(0.5) * wB.x};
+      if (!trig) {
+        Y[kk] = X.x;
+        if (kk > 0 && kk < N) Y[n - kk] = X.y;
+      } else {
+        const T c = twq[2 * kk], sn = -twq[2 * kk + 1];
+        const T v0 = T(2) * (X.x * c + X.y * sn);
+        const T v1 = T(2) * (X.x * sn - X.y * c);
+        Y[sine ? n - 1 - kk : kk] = v0;
+        if (kk > 0 && kk < N) Y[sine ? kk - 1 : n - kk] = v1;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// pre_inv from src (real lines) into dst (complex Z').
+template <typename T, int N, int L, int NT, int LS>
+__device__ __forceinline__ void pre_inv_pp(const T *src, T *dst, int kind,
+                                           const T *__restrict__ twh,
+                                           const T *__restrict__ twq) {
+  constexpr int NP = N / 2 + 1;
+  constexpr int TOT = L * NP;
+  const bool trig = is_trig(kind), sine = is_sine(kind);
+  for (int t = threadIdx.x; t < TOT; t += NT) {
+    const int l = t / NP, k = t - (t / NP) * NP;
+    const T *X = src + l * LS;
+    cx<T> *Z = reinterpret_cast<cx<T> *>(dst + l * LS);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kk = (h == 0) ? k : N - k;
+      if (h == 1 && (k == 0 || 2 * k == N)) continue;
+      cx<T> Xk, Xm;
+      if (!trig) {
+        Xk = hc<T, N>(X, kk);
+        Xm = hc<T, N>(X, N - kk);
+      } else {
+        Xk = vprime<T, N>(X, kk, twq, sine);
+        Xm = vprime<T, N>(X, N - kk, twq, sine);
+      }
+      const cx<T> A = add(Xk, cj(Xm)), B = sub(Xk, cj(Xm));
+      const cx<T> w = {twh[2 * kk], -twh[2 * kk + 1]};
+      Z[kk] = add(A, msi<T, 1>(mul(w, B)));
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace tile
+}  // namespace flutas

@@ -112,12 +112,12 @@ int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
 
 /* Per-stage timing of the solve pipeline (no counterpart in the reference,
  * whose profiler times the whole SOLVER region, src/profiler.f90:83-205):
- * runs `reps` solves on p and returns the mean duration in ms of each stage,
- * measured with HIP events on the library stream.  *nstages receives the
- * number of stages (<= max_stages); flutas_psolve_stage_name names them.
- * p is overwritten by each solve, so pass a scratch copy of the RHS. */
-int flutas_psolve_profile(flutas_psolve *h, double *p, int reps,
-                          int max_stages, double *ms, int *nstages);
+ * runs `reps` out-of-place solves f -> p and returns the mean duration in
+ * ms of each stage, measured with HIP events on the library stream.
+ * *nstages receives the number of stages (<= max_stages);
+ * flutas_psolve_stage_name names them. */
+int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
+                          int reps, int max_stages, double *ms, int *nstages);
 const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 
 /* fft (src/fft.f90:181-193): executes one r2r kind in place on `howmany`

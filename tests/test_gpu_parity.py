@@ -76,6 +76,12 @@ SOLVE_CASES = [
     ((48, 96, 30), (1.0, 1.0, 1.0), "PPPPNN", 1.5),   # radix-3 lengths
     ((40, 10, 14), (1.0, 1.0, 1.0), "DDPPDD", 0.0),   # radix-5 lengths
     ((14, 22, 12), (1.0, 1.0, 1.0), "PPNNNN", 0.0),   # generic radix 7, 11
+    # four-pass fused pipeline (compiled lengths in x and y, non-periodic z)
+    ((64, 32, 40), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((32, 64, 26), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((96, 48, 20), (1.0, 1.0, 1.0), "PPPPDD", 1.2),
+    ((128, 192, 18), (1.0, 2.0, 1.0), "DDNNND", 0.0),
+    ((256, 64, 34), (1.0, 1.0, 1.0), "NNPPDN", 1.1),
 ]
 
 
@@ -124,6 +130,26 @@ def test_fp64_solve_matches_oracle(api, oracle, ng, lengths, cbc, alpha):
         assert relmax(gauge_free(interior(got)), gauge_free(interior(pref))) <= 1e-12
     fd = api.DeviceArray.from_host(f)
     assert plan.residual(fd, d) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_fused_and_line_pipelines_agree(api, oracle, monkeypatch, precision):
+    """The four-pass pipeline and the five-pass line pipeline (forced with
+    FLUTAS_UNFUSED=1 at plan creation) solve the same problem alike."""
+    ng, lengths, cbc, alpha = (64, 48, 40), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5
+    _, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], alpha, lengths[2])
+    dli = tuple(n / l for n, l in zip(ng, lengths))
+    f = _rhs(ng, dzf, True, seed=4)
+    out = []
+    for unfused in ("0", "1"):
+        monkeypatch.setenv("FLUTAS_UNFUSED", unfused)
+        plan = api.PoissonPlan(ng, dli, dzci, dzfi, cbc, precision=precision, tol=1e-10)
+        d = api.DeviceArray.from_host(f)
+        plan.solve(d)
+        out.append(interior(d.to_host()))
+        plan.destroy()
+    tol = 1e-13 if precision == 64 else 1e-8
+    assert relmax(gauge_free(out[0]), gauge_free(out[1])) <= tol
 
 
 def _fields(ng, seed, nh_u=1):

@@ -388,10 +388,12 @@ int LineTables::build(int n_) {
 
 void LineTables::release() {
   for (void *p : {(void *)d_tw, (void *)d_twh, (void *)d_twq, (void *)f_tw,
-                  (void *)f_twh, (void *)f_twq})
+                  (void *)f_twh, (void *)f_twq, (void *)d_ptw, (void *)f_ptw})
     if (p) (void)hipFree(p);
   d_tw = d_twh = d_twq = nullptr;
   f_tw = f_twh = f_twq = nullptr;
+  d_ptw = nullptr;
+  f_ptw = nullptr;
 }
 
 template <typename T, typename Tin, typename Tout>

@@ -225,6 +225,10 @@ __global__ void __launch_bounds__(256)
 // L is the operator the spectral solve inverts: second differences in x, y
 // with homogeneous ghost rules (P wrap, N p0 = p1, D p0 = -p1) and the
 // tridiagonal rows a, b, c in z (periodic wrap for PP).
+// One thread walks one (i, j) column through kResidualPlanes planes,
+// keeping the z-neighbours in registers; one partial max per workgroup.
+constexpr int kResidualPlanes = 32;
+
 template <typename Tr>
 __global__ void __launch_bounds__(256)
     k_residual(OperatorDesc op, const double *__restrict__ f,
@@ -234,32 +238,38 @@ __global__ void __launch_bounds__(256)
   __shared__ double smax[256];
   const int nx = op.nx, ny = op.ny, nz = op.nz;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  const int j = blockIdx.y;
+  const int k0 = blockIdx.z * kResidualPlanes;
+  const int k1 = min(nz, k0 + kResidualPlanes);
   double am = 0.0;
   if (i < nx) {
     const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
-    const long long c = (i + 1) + sy * (j + 1) + sz * (k + 1);
-    const double pc = p[c];
-    double pm, pp;
-    if (i > 0) pm = p[c - 1];
-    else pm = (op.cbc[0] == 'P') ? p[c + (nx - 1)] : (op.cbc[0] == 'N') ? pc : -pc;
-    if (i < nx - 1) pp = p[c + 1];
-    else pp = (op.cbc[1] == 'P') ? p[c - (nx - 1)] : (op.cbc[1] == 'N') ? pc : -pc;
-    const double lx = (pp - 2.0 * pc + pm) * op.dx2;
-    if (j > 0) pm = p[c - sy];
-    else pm = (op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : (op.cbc[2] == 'N') ? pc : -pc;
-    if (j < ny - 1) pp = p[c + sy];
-    else pp = (op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : (op.cbc[3] == 'N') ? pc : -pc;
-    const double ly = (pp - 2.0 * pc + pm) * op.dy2;
-    double lz = op.b[k] * pc;
-    if (k > 0) lz += op.a[k] * p[c - sz];
-    else if (op.cbc[4] == 'P') lz += op.a[k] * p[c + sz * (nz - 1)];
-    if (k < nz - 1) lz += op.c[k] * p[c + sz];
-    else if (op.cbc[5] == 'P') lz += op.c[k] * p[c - sz * (nz - 1)];
-    const double res = f[c] - ((lx + ly) + lz);
-    if (r) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
-    am = fabs(res);
-    if (res != res) am = res;
+    long long c = (i + 1) + sy * (j + 1) + sz * (k0 + 1);
+    double pc = p[c];
+    double pzm = (k0 > 0) ? p[c - sz] : ((op.cbc[4] == 'P') ? p[c + sz * (nz - 1)] : 0.0);
+    for (int k = k0; k < k1; ++k, c += sz) {
+      double pm, pp;
+      if (i > 0) pm = p[c - 1];
+      else pm = (op.cbc[0] == 'P') ? p[c + (nx - 1)] : (op.cbc[0] == 'N') ? pc : -pc;
+      if (i < nx - 1) pp = p[c + 1];
+      else pp = (op.cbc[1] == 'P') ? p[c - (nx - 1)] : (op.cbc[1] == 'N') ? pc : -pc;
+      const double lx = (pp - 2.0 * pc + pm) * op.dx2;
+      if (j > 0) pm = p[c - sy];
+      else pm = (op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : (op.cbc[2] == 'N') ? pc : -pc;
+      if (j < ny - 1) pp = p[c + sy];
+      else pp = (op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : (op.cbc[3] == 'N') ? pc : -pc;
+      const double ly = (pp - 2.0 * pc + pm) * op.dy2;
+      const double pzp = (k < nz - 1) ? p[c + sz] : ((op.cbc[5] == 'P') ? p[c - sz * (nz - 1)] : 0.0);
+      double lz = op.b[k] * pc;
+      if (k > 0 || op.cbc[4] == 'P') lz += op.a[k] * pzm;
+      if (k < nz - 1 || op.cbc[5] == 'P') lz += op.c[k] * pzp;
+      const double res = f[c] - ((lx + ly) + lz);
+      if (r) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
+      const double ar = fabs(res);
+      if (ar > am || res != res) am = (res != res) ? res : ar;
+      pzm = pc;
+      pc = pzp;
+    }
   }
   smax[threadIdx.x] = am;
   __syncthreads();
@@ -279,12 +289,16 @@ __global__ void __launch_bounds__(256)
              double *__restrict__ part) {
   __shared__ double smax[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  const int j = blockIdx.y;
+  const int k0 = blockIdx.z * kResidualPlanes;
+  const int k1 = min(nz, k0 + kResidualPlanes);
   double am = 0.0;
   if (i < nx) {
-    const double v = f[(i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1))];
-    am = fabs(v);
-    if (v != v) am = v;
+    for (int k = k0; k < k1; ++k) {
+      const double v = f[(i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1))];
+      const double av = fabs(v);
+      if (av > am || v != v) am = (v != v) ? v : av;
+    }
   }
   smax[threadIdx.x] = am;
   __syncthreads();
@@ -317,13 +331,40 @@ int launch_accumulate(const float *r, double *p, int nx, int ny, int nz,
   return FLUTAS_OK;
 }
 
+// max over n partial maxima (NaN propagates), one workgroup
+__global__ void __launch_bounds__(256)
+    k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out) {
+  __shared__ double smax[256];
+  double m = 0.0;
+  for (int t = threadIdx.x; t < n; t += 256) {
+    const double o = part[t];
+    if (o > m || o != o) m = o;
+  }
+  smax[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double o = smax[threadIdx.x + off];
+      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = smax[0];
+}
+
+int launch_reduce_max(const double *part, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, s, part, n, out);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
 static int xblock(int nx) { return std::min(256, ((nx + 63) / 64) * 64); }
 
 template <typename Tr>
 int launch_residual(const OperatorDesc &op, const double *f, const double *p,
                     Tr *r, double *partial, int *nparts, hipStream_t s) {
   const int bx = 256;
-  dim3 grid((op.nx + bx - 1) / bx, op.ny, op.nz);
+  dim3 grid((op.nx + bx - 1) / bx, op.ny, (op.nz + kResidualPlanes - 1) / kResidualPlanes);
   hipLaunchKernelGGL(k_residual<Tr>, grid, dim3(bx), 0, s, op, f, p, r, partial);
   FL_HIP(hipGetLastError());
   *nparts = (int)(grid.x * grid.y * grid.z);
@@ -335,7 +376,7 @@ template int launch_residual<double>(const OperatorDesc &, const double *, const
 int launch_absmax(const double *f, int nx, int ny, int nz, double *partial,
                   int *nparts, hipStream_t s) {
   const int bx = 256;
-  dim3 grid((nx + bx - 1) / bx, ny, nz);
+  dim3 grid((nx + bx - 1) / bx, ny, (nz + kResidualPlanes - 1) / kResidualPlanes);
   hipLaunchKernelGGL(k_absmax, grid, dim3(bx), 0, s, f, nx, ny, nz, partial);
   FL_HIP(hipGetLastError());
   *nparts = (int)(grid.x * grid.y * grid.z);

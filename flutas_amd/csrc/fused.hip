@@ -42,6 +42,51 @@ constexpr int tile_lines() {
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
 
+// Twiddle tables staged in LDS once per workgroup: exp(-2 pi i m/N) (N),
+// exp(-2 pi i k/n) (N+1) and exp(-i pi k/2n) (N+1, the entries the real
+// split and the cosine/sine phases use).  Keeping them out of global memory
+// leaves the vector-memory counter to the field loads, so a prefetch is not
+// drained by the first twiddle read.
+template <typename T, int NC>
+struct Tw {
+  static constexpr int ntw = 2 * tile::Plan<NC>::tw_total();
+  static constexpr int nth = 2 * (NC + 1), ntq = 2 * (NC + 1);
+  static constexpr int total = ntw + nth + ntq;
+  const T *tw, *th, *tq;
+};
+template <typename T, int NC, int NT>
+__device__ __forceinline__ Tw<T, NC> stage_twiddles(T *s, const T *__restrict__ tw,
+                                                    const T *__restrict__ twh,
+                                                    const T *__restrict__ twq) {
+  using W = Tw<T, NC>;
+  // fixed trip counts with clamped indices: the loads are issued back to
+  // back without branches, then stored to LDS
+  constexpr int Q1 = (W::ntw + NT - 1) / NT, Q2 = (W::nth + NT - 1) / NT;
+  T v1[Q1], v2[Q2], v3[Q2];
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) v1[q] = tw[min((int)threadIdx.x + q * NT, W::ntw - 1)];
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) {
+    const int t = min((int)threadIdx.x + q * NT, W::nth - 1);
+    v2[q] = twh[t];
+    v3[q] = twq[t];
+  }
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < W::ntw) s[t] = v1[q];
+  }
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < W::nth) {
+      s[W::ntw + t] = v2[q];
+      s[W::ntw + W::nth + t] = v3[q];
+    }
+  }
+  return Tw<T, NC>{s, s + W::ntw, s + W::ntw + W::nth};
+}
+
 // ---------------------------------------------------------------- P1 -----
 // One workgroup: TJ consecutive x-lines (j0..j0+TJ-1) at one k.
 // in: element (i, j, k) at in[j*is1 + k*is2 + i] (i, j, k 0-based).
@@ -60,13 +105,12 @@ __global__ void __launch_bounds__(NT)
   const int nl = min(TJ, ny - j0);
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
   Tin r[LD];
+  // branch-free loads: lines past ny (partial tile) re-read line ny-1
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
     const int l = t / n, e = t - (t / n) * n;
-    r[q] = (t < TJ * n && l < nl)
-               ? in[(long long)(j0 + l) * is1 + (long long)k * is2 + e]
-               : Tin(0);
+    r[q] = in[(long long)min(j0 + l, ny - 1) * is1 + (long long)k * is2 + e];
   }
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
@@ -81,9 +125,11 @@ __global__ void __launch_bounds__(NT)
     }
     sm[l * LS + pos] = x;
   }
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   __syncthreads();
-  tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), tw);
-  tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, twh, twq);
+  tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
+  tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
   // transposed store
   T *plane = Wt + (long long)k * n * ny;
 #pragma unroll
@@ -113,27 +159,27 @@ __global__ void __launch_bounds__(NT)
   T r[LD];
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
-    const int t = threadIdx.x + q * NT;
+    const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
     const int l = t % TJ, i = t / TJ;
-    r[q] = (t < TJ * n && l < nl) ? plane[(long long)i * ny + j0 + l] : T(0);
+    r[q] = plane[(long long)i * ny + min(j0 + l, ny - 1)];
   }
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int t = threadIdx.x + q * NT;
     if (t < TJ * n) sm[(t % TJ) * LS + t / TJ] = r[q];
   }
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   __syncthreads();
-  tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, twh, twq);
-  tile::cfft<T, +1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), tw);
+  tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  tile::cfft<T, +1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
   Tout prev[LD];
   if (accumulate) {
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
-      const int t = threadIdx.x + q * NT;
+      const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
       const int l = t / n, e = t - (t / n) * n;
-      prev[q] = (t < TJ * n && l < nl)
-                    ? out[(long long)(j0 + l) * os1 + (long long)k * os2 + e]
-                    : Tout(0);
+      prev[q] = out[(long long)min(j0 + l, ny - 1) * os1 + (long long)k * os2 + e];
     }
   }
 #pragma unroll
@@ -176,6 +222,8 @@ __global__ void __launch_bounds__(NT)
   constexpr bool fft_in_b = (tile::Plan<NC>::nst % 2) == 1;
   T *const Z = fft_in_b ? B : A;  // complex FFT result
   T *const Y = fft_in_b ? A : B;  // spectral lines after the real split
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int i = blockIdx.x;
   const int ngrp = (nz + GP - 1) / GP;
   const long long pstride = (long long)nx * n;  // plane stride
@@ -191,13 +239,14 @@ __global__ void __launch_bounds__(NT)
     pprev[u] = 0.0;
   }
   T r[LD];
+  // branch-free prefetch: planes past nz (last group) re-read plane nz-1
   auto load = [&](int g) {
     const int k0 = g * GP;
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
-      const int t = threadIdx.x + q * NT;
+      const int t = min((int)threadIdx.x + q * NT, GP * n - 1);
       const int l = t / n, e = t - (t / n) * n;
-      r[q] = (t < GP * n && k0 + l < nz) ? line0[(long long)(k0 + l) * pstride + e] : T(0);
+      r[q] = line0[(long long)min(k0 + l, nz - 1) * pstride + e];
     }
   };
   load(0);
@@ -220,8 +269,8 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     if (g + 1 < ngrp) load(g + 1);
     tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
-                                            reinterpret_cast<cx<T> *>(B), tw);
-    tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, twh, twq);
+                                            reinterpret_cast<cx<T> *>(B), W.tw);
+    tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
     // forward elimination, plane by plane (dgtsv_homebrewed order)
     for (int l = 0; l < np; ++l) {
       const int k = k0 + l;
@@ -280,6 +329,8 @@ __global__ void __launch_bounds__(NT)
   // pre_inv A -> B, then the FFT passes start in B
   constexpr bool fft_in_a = (tile::Plan<NC>::nst % 2) == 1;
   T *const R = fft_in_a ? A : B;  // inverse FFT result
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int i = blockIdx.x;
   const int ngrp = (nz + GP - 1) / GP;
   const long long pstride = (long long)nx * n;
@@ -298,17 +349,18 @@ __global__ void __launch_bounds__(NT)
   double dck[CPT];
   auto load = [&](int g) {
     const int k0 = g * GP;
+    // branch-free: out-of-range planes/columns re-read valid entries
 #pragma unroll
     for (int l = 0; l < GP; ++l)
 #pragma unroll
       for (int u = 0; u < CPT; ++u) {
-        const int j = threadIdx.x + u * NT;
-        pv[l][u] = (j < n && k0 + l < nz) ? line0[(long long)(k0 + l) * pstride + j] : T(0);
+        const int j = min((int)threadIdx.x + u * NT, n - 1);
+        pv[l][u] = line0[(long long)min(k0 + l, nz - 1) * pstride + j];
       }
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
-      const int j = threadIdx.x + u * NT;
-      dck[u] = (g > 0 && j < n) ? ck[((long long)i * ngrp + g - 1) * n + j] : 0.0;
+      const int j = min((int)threadIdx.x + u * NT, n - 1);
+      dck[u] = ck[((long long)i * ngrp + max(g - 1, 0)) * n + j];
     }
   };
   load(ngrp - 1);
@@ -344,9 +396,9 @@ __global__ void __launch_bounds__(NT)
     }
     __syncthreads();
     if (g > 0) load(g - 1);
-    tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, twh, twq);
+    tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
     tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
-                                            reinterpret_cast<cx<T> *>(A), tw);
+                                            reinterpret_cast<cx<T> *>(A), W.tw);
     constexpr int LD = (GP * n + NT - 1) / NT;
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
@@ -394,9 +446,48 @@ bool fused_supported(int n) {
     default: return fail(FLUTAS_EUNSUPPORTED, "no fused kernel for this length"); \
   }
 
+// The fused kernels use the per-pass twiddle tables in place of the plain
+// exp(-2 pi i m/N) table.
 template <typename T>
 static LinePlan<T> plan_of(const LineTables &t) {
-  if constexpr (sizeof(T) == 8) return t.dplan(); else return t.fplan();
+  LinePlan<T> p;
+  if constexpr (sizeof(T) == 8) { p = t.dplan(); p.tw = t.d_ptw; }
+  else { p = t.fplan(); p.tw = t.f_ptw; }
+  return p;
+}
+
+// Per-pass twiddles of Plan<NC>: pass s >= 1, entry (rr, j) =
+// exp(-2 pi i rr j / (ns(s) r[s])), laid out [rr][j].
+template <int NC>
+static void pass_twiddles(std::vector<double> &out) {
+  using P = tile::Plan<NC>;
+  const long double pi = 3.14159265358979323846264338327950288L;
+  out.assign(2 * P::tw_total(), 0.0);
+  for (int s = 1; s < P::nst; ++s) {
+    const int R = P::r[s], NS = P::ns(s);
+    for (int rr = 0; rr < R; ++rr)
+      for (int j = 0; j < NS; ++j) {
+        const int m = P::tw_off(s) + rr * NS + j;
+        const long double a = -2.0L * pi * (long double)(rr * j) / (long double)(NS * R);
+        out[2 * m] = (double)cosl(a);
+        out[2 * m + 1] = (double)sinl(a);
+      }
+  }
+}
+
+int build_fused_twiddles(LineTables &tab) {
+  std::vector<double> h;
+  FL_DISPATCH(tab.n / 2, pass_twiddles<NC>(h));
+  if (tab.d_ptw) (void)hipFree(tab.d_ptw);
+  if (tab.f_ptw) (void)hipFree(tab.f_ptw);
+  tab.d_ptw = nullptr;
+  tab.f_ptw = nullptr;
+  FL_HIP(hipMalloc((void **)&tab.d_ptw, h.size() * sizeof(double)));
+  FL_HIP(hipMemcpy(tab.d_ptw, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<float> f(h.begin(), h.end());
+  FL_HIP(hipMalloc((void **)&tab.f_ptw, f.size() * sizeof(float)));
+  FL_HIP(hipMemcpy(tab.f_ptw, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+  return FLUTAS_OK;
 }
 
 template <typename T, typename Tin>
@@ -430,12 +521,13 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
   return FLUTAS_OK;
 }
 
-// P2/P3 plane groups: half the P1 tile so that two slab workgroups (and
-// the register prefetch) fit one CU.
+// P2/P3 plane groups: the two ping-pong tiles of a group stay within
+// kTileBytes so that two slab workgroups fit one CU.
 template <typename T, int NC>
 constexpr int group_planes() {
-  constexpr int l = tile_lines<T, NC>() / 2;
-  return l >= 1 ? l : 1;
+  constexpr int bytes = 2 * line_stride<T, 2 * NC>() * (int)sizeof(T);
+  constexpr int l = kTileBytes / bytes;
+  return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
 
 int fused_group_planes(int ny, int elem_bytes) {

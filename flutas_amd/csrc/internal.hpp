@@ -61,6 +61,10 @@ struct LineTables {
   int radix[kMaxStages] = {};
   double *d_tw = nullptr, *d_twh = nullptr, *d_twq = nullptr;
   float *f_tw = nullptr, *f_twh = nullptr, *f_twq = nullptr;
+  // per-pass twiddle tables of the fused kernels (set by
+  // build_fused_twiddles when the length has a compiled plan)
+  double *d_ptw = nullptr;
+  float *f_ptw = nullptr;
   LinePlan<double> dplan() const;
   LinePlan<float> fplan() const;
   int build(int n);
@@ -97,6 +101,8 @@ int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
 // Work layout Wt[i][k][j]; see fused.hip.
 bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
+// Uploads the per-pass twiddle tables of the compiled plan of tab.n.
+int build_fused_twiddles(LineTables &tab);
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  long long is2, T *Wt, int ny, int nz, hipStream_t s);
@@ -132,5 +138,7 @@ int launch_accumulate(const float *r, double *p_halo, int nx, int ny, int nz,
                       hipStream_t s);
 int launch_absmax(const double *f_halo, int nx, int ny, int nz, double *partial,
                   int *nparts, hipStream_t s);
+// out[0] = max of n partial maxima (single workgroup)
+int launch_reduce_max(const double *part, int n, double *out, hipStream_t s);
 
 }  // namespace flutas

@@ -301,14 +301,12 @@ OperatorDesc op_desc(const flutas_psolve *h) {
   return op;
 }
 
+// Device-side final reduction of the partial maxima; one 8-byte copy back.
 int reduce_max(flutas_psolve *h, int nparts, hipStream_t s, double *out) {
-  std::vector<double> buf(nparts);
-  FL_HIP(hipMemcpyAsync(buf.data(), h->part, sizeof(double) * nparts, hipMemcpyDeviceToHost, s));
+  double *res = h->part + h->npart;
+  FL_TRY(launch_reduce_max(h->part, nparts, res, s));
+  FL_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, s));
   FL_HIP(hipStreamSynchronize(s));
-  double m = 0.0;
-  for (double v : buf)
-    if (v > m || v != v) m = v;
-  *out = m;
   return FLUTAS_OK;
 }
 
@@ -453,6 +451,8 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
   do {
     if ((rc = h->tx.build(h->nx))) break;
     if ((rc = h->ty.build(h->ny))) break;
+    if (fused_supported(h->nx) && (rc = build_fused_twiddles(h->tx))) break;
+    if (fused_supported(h->ny) && (rc = build_fused_twiddles(h->ty))) break;
     if ((rc = dev_upload(&h->d_lamx, h->lamx))) break;
     if ((rc = dev_upload(&h->d_lamy, h->lamy))) break;
     if ((rc = dev_upload(&h->d_a, h->a))) break;
@@ -482,7 +482,7 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
       if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
     }
     h->npart = ((h->nx + 255) / 256) * (size_t)h->ny * h->nz;
-    if (e == hipSuccess) e = hipMalloc((void **)&h->part, h->npart * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->part, (h->npart + 1) * sizeof(double));
     if (e != hipSuccess) { rc = fail(FLUTAS_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e)); break; }
   } while (0);
   if (rc) {

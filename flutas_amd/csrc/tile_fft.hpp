@@ -128,14 +128,21 @@ __device__ __forceinline__ void dft(cx<T> *v) {
   else static_assert(R == 2, "radix not built");
 }
 
-// Compile-time radix plans of the complex length N = n/2.
+// Compile-time radix plans of the complex length N = n/2.  Pass s reads
+// with stride N/r[s] and writes with stride ns(s) = r[0]*...*r[s-1].
+// Twiddles of pass s >= 1 live in their own table laid out [rr][j]
+// (rr < r[s], j < ns(s)) so that lanes with consecutive j read
+// consecutive entries; tw_off(s) is the table's offset in complex values.
 template <int N>
 struct Plan;
-#define FL_PLAN(NN, A, B, C, NS)                \
-  template <>                                   \
-  struct Plan<NN> {                             \
-    static constexpr int nst = NS;              \
-    static constexpr int r[3] = {A, B, C};      \
+#define FL_PLAN(NN, A, B, C, NS)                                        \
+  template <>                                                           \
+  struct Plan<NN> {                                                     \
+    static constexpr int nst = NS;                                      \
+    static constexpr int r[3] = {A, B, C};                              \
+    static constexpr int ns(int s) { return s == 0 ? 1 : (s == 1 ? A : A * B); } \
+    static constexpr int tw_off(int s) { return s <= 1 ? 0 : B * A; }   \
+    static constexpr int tw_total() { return NS == 1 ? 1 : (NS == 2 ? B * A : B * A + C * A * B); } \
   };
 FL_PLAN(16, 16, 1, 1, 1)
 FL_PLAN(24, 8, 3, 1, 2)
@@ -151,9 +158,47 @@ FL_PLAN(512, 8, 8, 8, 3)
 FL_PLAN(1024, 16, 16, 4, 3)
 #undef FL_PLAN
 
-// One Stockham pass of radix R over L lines (complex line stride LSC).
-template <typename T, int S, int N, int R, int NS, int L, int NT, int LSC>
+// XOR swizzle of the buffer between passes 0 and 1 (power-of-two r[0]):
+// pass 0 writes element b*r0 + rr of butterfly b; swizzling the low bits by
+// b spreads the r0 lanes of a write over distinct LDS banks, and pass 1's
+// contiguous reads stay within one block of r0 elements.
+template <int N>
+__device__ __forceinline__ constexpr bool swizzled() {
+  constexpr int r0 = Plan<N>::r[0];
+  return Plan<N>::nst >= 2 && (r0 & (r0 - 1)) == 0;
+}
+template <int N>
+__device__ __forceinline__ int swz(int m) {
+  constexpr int r0 = Plan<N>::r[0];
+  return m ^ ((m / r0) & (r0 - 1));
+}
+template <int N, int SI>
+__device__ __forceinline__ int in_idx(int m) {
+  if constexpr (SI == 1 && swizzled<N>()) return swz<N>(m);
+  return m;
+}
+template <int N, int SI>
+__device__ __forceinline__ int out_idx(int m) {
+  if constexpr (SI == 0 && swizzled<N>()) return swz<N>(m);
+  return m;
+}
+
+// Twiddle of pass SI (>= 1) for butterfly position j and input rr, from the
+// per-pass table (see Plan).
+template <typename T, int S, int N, int SI>
+__device__ __forceinline__ cx<T> ptw(const T *__restrict__ tw, int rr, int j) {
+  const int m = Plan<N>::tw_off(SI) + rr * Plan<N>::ns(SI) + j;
+  cx<T> w = {tw[2 * m], tw[2 * m + 1]};
+  if (S > 0) w.y = -w.y;
+  return w;
+}
+
+// Stockham pass SI of radix R = Plan<N>::r[SI] over L lines (complex line
+// stride LSC), in place: all inputs are read into registers, then written.
+template <typename T, int S, int N, int SI, int L, int NT, int LSC>
 __device__ __forceinline__ void pass(cx<T> *buf, const T *__restrict__ tw) {
+  constexpr int R = Plan<N>::r[SI];
+  constexpr int NS = Plan<N>::ns(SI);
   constexpr int NB = N / R;
   constexpr int TOT = L * NB;
   constexpr int PER = (TOT + NT - 1) / NT;
@@ -168,13 +213,8 @@ __device__ __forceinline__ void pass(cx<T> *buf, const T *__restrict__ tw) {
       const cx<T> *x = buf + l * LSC;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        cx<T> a = x[b + r * NB];
-        if (NS > 1 && r > 0) {
-          const int m = r * j * (N / (NS * R));
-          cx<T> w = {tw[2 * m], tw[2 * m + 1]};
-          if (S > 0) w.y = -w.y;
-          a = mul(a, w);
-        }
+        cx<T> a = x[in_idx<N, SI>(b + r * NB)];
+        if (SI > 0 && r > 0) a = mul(a, ptw<T, S, N, SI>(tw, r, j));
         v[u][r] = a;
       }
     }
@@ -190,20 +230,19 @@ __device__ __forceinline__ void pass(cx<T> *buf, const T *__restrict__ tw) {
       cx<T> *y = buf + l * LSC;
       const int base = (b / NS) * NS * R + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) y[base + r * NS] = v[u][r];
+      for (int r = 0; r < R; ++r) y[out_idx<N, SI>(base + r * NS)] = v[u][r];
     }
   }
   __syncthreads();
 }
 
-// Complex FFT of length N on L lines, sign S; tw = exp(-2 pi i m / N).
+// Complex FFT of length N on L lines, sign S; tw = per-pass twiddle tables.
 template <typename T, int S, int N, int L, int NT, int LSC>
 __device__ __forceinline__ void cfft(cx<T> *buf, const T *__restrict__ tw) {
   using P = Plan<N>;
-  pass<T, S, N, P::r[0], 1, L, NT, LSC>(buf, tw);
-  if constexpr (P::nst > 1) pass<T, S, N, P::r[1], P::r[0], L, NT, LSC>(buf, tw);
-  if constexpr (P::nst > 2)
-    pass<T, S, N, P::r[2], P::r[0] * P::r[1], L, NT, LSC>(buf, tw);
+  pass<T, S, N, 0, L, NT, LSC>(buf, tw);
+  if constexpr (P::nst > 1) pass<T, S, N, 1, L, NT, LSC>(buf, tw);
+  if constexpr (P::nst > 2) pass<T, S, N, 2, L, NT, LSC>(buf, tw);
 }
 
 __device__ __forceinline__ bool is_fwd(int kind) {
@@ -368,9 +407,11 @@ __device__ __forceinline__ void pre_inv(T *sm, int kind, const T *__restrict__ t
 // barrier at the end; no register staging (lower register pressure at the
 // cost of a second LDS tile).
 
-template <typename T, int S, int N, int R, int NS, int L, int NT, int LSC>
+template <typename T, int S, int N, int SI, int L, int NT, int LSC>
 __device__ __forceinline__ void pass_pp(const cx<T> *src, cx<T> *dst,
                                         const T *__restrict__ tw) {
+  constexpr int R = Plan<N>::r[SI];
+  constexpr int NS = Plan<N>::ns(SI);
   constexpr int NB = N / R;
   constexpr int TOT = L * NB;
   for (int t = threadIdx.x; t < TOT; t += NT) {
@@ -380,20 +421,15 @@ __device__ __forceinline__ void pass_pp(const cx<T> *src, cx<T> *dst,
     cx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      cx<T> a = x[b + r * NB];
-      if (NS > 1 && r > 0) {
-        const int m = r * j * (N / (NS * R));
-        cx<T> w = {tw[2 * m], tw[2 * m + 1]};
-        if (S > 0) w.y = -w.y;
-        a = mul(a, w);
-      }
+      cx<T> a = x[in_idx<N, SI>(b + r * NB)];
+      if (SI > 0 && r > 0) a = mul(a, ptw<T, S, N, SI>(tw, r, j));
       v[r] = a;
     }
     dft<T, S, R>(v);
     cx<T> *y = dst + l * LSC;
     const int base = (b / NS) * NS * R + j;
 #pragma unroll
-    for (int r = 0; r < R; ++r) y[base + r * NS] = v[r];
+    for (int r = 0; r < R; ++r) y[out_idx<N, SI>(base + r * NS)] = v[r];
   }
   __syncthreads();
 }
@@ -403,10 +439,9 @@ __device__ __forceinline__ void pass_pp(const cx<T> *src, cx<T> *dst,
 template <typename T, int S, int N, int L, int NT, int LSC>
 __device__ __forceinline__ void cfft_pp(cx<T> *a, cx<T> *b, const T *__restrict__ tw) {
   using P = Plan<N>;
-  pass_pp<T, S, N, P::r[0], 1, L, NT, LSC>(a, b, tw);
-  if constexpr (P::nst > 1) pass_pp<T, S, N, P::r[1], P::r[0], L, NT, LSC>(b, a, tw);
-  if constexpr (P::nst > 2)
-    pass_pp<T, S, N, P::r[2], P::r[0] * P::r[1], L, NT, LSC>(a, b, tw);
+  pass_pp<T, S, N, 0, L, NT, LSC>(a, b, tw);
+  if constexpr (P::nst > 1) pass_pp<T, S, N, 1, L, NT, LSC>(b, a, tw);
+  if constexpr (P::nst > 2) pass_pp<T, S, N, 2, L, NT, LSC>(a, b, tw);
 }
 
 // post_fwd from src (complex FFT of the packed lines) into dst.

@@ -34,9 +34,18 @@ def log(msg):
 
 
 # Algorithmic HBM bytes per grid point of each pipeline stage: one read and
-# one write of the field at the stage's storage width.
-STAGE_BYTES_FP64 = {"x_fwd": 16, "y_fwd": 16, "z_solve": 16, "y_bwd": 16, "x_bwd": 16}
+# one write of the field at FP64 width.  The whole solve is priced on the
+# five-sweep pencil model (x-fwd, y-fwd, z-solve, y-bwd, x-bwd): 80 B/pt.
+STAGE_BYTES_FP64 = 16
 SOLVE_BYTES_FP64 = 80
+# stage name -> kernel (as rocprofv3 names it) that the stage launches
+STAGE_KERNEL = {"P1_x_fwd": "k_x_fwd<double", "P2_yz_fwd": "k_yz_fwd<double",
+                "P3_zy_bwd": "k_zy_bwd<double", "P4_x_bwd": "k_x_bwd<double"}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_traffic_fused_fp64_512_channel.txt")
+# Plain copy used to calibrate the rocprofv3 byte counters: the interior of a
+# 512^3 halo-1 field (510^3 FP64 points) read once and written once.
+COPY_N = 510
+COPY_BYTES = COPY_N ** 3 * 8
 
 
 def time_steps(fn, steps, warmup, api):
@@ -91,6 +100,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run the plain copy the byte-counter summaries calibrate on")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; run it with --gpus 1 "
@@ -113,12 +124,21 @@ def main():
     ms, wall_ms = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
     relres64 = plan.residual(f, p)
     log(f"bench: fp64 {ms:.3f} ms/solve (wall {wall_ms:.3f}), relres {relres64:.2e}")
+    # per-stage kernel times: HIP events between the stages on the library
+    # stream, over profile_reps solves run back to back
     stages = plan.profile(f, p, args.profile_reps)
     stage_rows = []
     for name, t in stages:
-        b = STAGE_BYTES_FP64.get(name, 16) * npts
+        b = STAGE_BYTES_FP64 * npts
         stage_rows.append({"stage": name, "ms": t, "GBps": b / (t * 1e6)})
     dom = max(stage_rows, key=lambda r: r["ms"])
+    traffic = None
+    if os.path.exists(TRAFFIC_SUMMARY) and dom["stage"] in STAGE_KERNEL:
+        from flutas_amd.profsum import read_traffic
+        key = STAGE_KERNEL[dom["stage"]]
+        for kname, rec in read_traffic(TRAFFIC_SUMMARY).items():
+            if kname.startswith(key):
+                traffic = rec["total"]
     log("bench: stages " + ", ".join(f"{r['stage']} {r['ms']:.3f} ms ({r['GBps']:.0f} GB/s)"
                                      for r in stage_rows))
     solve_gbps = SOLVE_BYTES_FP64 * npts / (ms * 1e6)
@@ -163,13 +183,24 @@ def main():
                    "ng": list(case.ng), "bc": case.cbc},
         "roofline": {"bound": "hbm", "kernel": dom["stage"],
                      "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None},
+                     "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": STAGE_BYTES_FP64 * npts,
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT) if traffic else None},
         "solve_roofline": {"bytes_per_point": SOLVE_BYTES_FP64, "achieved": solve_gbps,
                            "frac": solve_gbps / HBM_PEAK_GBS},
         "stages": stage_rows,
         "fp64_relres": relres64,
         "fp32_refine": refine,
     }
+    if args.calibrate_copy:
+        del f, p
+        a = api.DeviceArray((COPY_N + 2,) * 3)
+        b = api.DeviceArray((COPY_N + 2,) * 3)
+        a.zero()
+        for _ in range(2):
+            api.copy_interior(COPY_N, COPY_N, COPY_N, a, b)
+        api.synchronize()
+        log(f"bench: calibration copy of {COPY_BYTES} B (interior of {COPY_N + 2}^3)")
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")
         result["cpu_baseline"] = cpu_baseline(case)

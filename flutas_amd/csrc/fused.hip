@@ -202,7 +202,15 @@ __global__ void __launch_bounds__(NT)
 
 // ---------------------------------------------------------------- P2 -----
 // One workgroup per x-slab i; planes are processed GP at a time.
-// ck[(i*ngrp + g)*ny + j] = d of the last plane of group g.
+// Checkpoints: ck[(i*nck + g)*ny + j] = d of the last plane of group g
+// (nck = number of groups); P3 restarts the recurrence from them.  (One
+// checkpoint per group: replaying a group of divides in P3 costs more
+// latency than the 1/GP extra traffic.)
+template <int GP>
+__host__ __device__ constexpr int checkpoints(int nz) {
+  return (nz + GP - 1) / GP;
+}
+
 template <typename T, int NC, int GP, int NT>
 __global__ void __launch_bounds__(NT)
     k_yz_fwd(T *__restrict__ Wt, double *__restrict__ ck, int nx, int nz,
@@ -226,6 +234,7 @@ __global__ void __launch_bounds__(NT)
   const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int i = blockIdx.x;
   const int ngrp = (nz + GP - 1) / GP;
+  const int nck = checkpoints<GP>(nz);
   const long long pstride = (long long)nx * n;  // plane stride
   T *line0 = Wt + (long long)i * n;             // plane 0 line of slab i
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
@@ -302,10 +311,11 @@ __global__ void __launch_bounds__(NT)
         line[j] = (T)pv;
       }
     }
+    // checkpoint after every group: ck[(i*nck + g)*ny + j]
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int j = threadIdx.x + u * NT;
-      if (j < n) ck[((long long)i * ngrp + g) * n + j] = dprev[u];
+      if (j < n) ck[((long long)i * nck + g) * n + j] = dprev[u];
     }
     __syncthreads();
   }
@@ -333,6 +343,7 @@ __global__ void __launch_bounds__(NT)
   const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int i = blockIdx.x;
   const int ngrp = (nz + GP - 1) / GP;
+  const int nck = checkpoints<GP>(nz);
   const long long pstride = (long long)nx * n;
   T *line0 = Wt + (long long)i * n;
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
@@ -357,10 +368,13 @@ __global__ void __launch_bounds__(NT)
         const int j = min((int)threadIdx.x + u * NT, n - 1);
         pv[l][u] = line0[(long long)min(k0 + l, nz - 1) * pstride + j];
       }
+    // the recurrence restarts at the end of group g - 1 (unused, but read
+    // in bounds, for g = 0)
+    const int c0 = max(g - 1, 0);
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int j = min((int)threadIdx.x + u * NT, n - 1);
-      dck[u] = ck[((long long)i * ngrp + max(g - 1, 0)) * n + j];
+      dck[u] = ck[((long long)i * nck + c0) * n + j];
     }
   };
   load(ngrp - 1);

@@ -391,6 +391,20 @@ int flutas_synchronize(void) {
   FL_HIP(hipStreamSynchronize(g_stream));
   return FLUTAS_OK;
 }
+int flutas_pointer_is_device(const void *ptr, int *is_device) {
+  if (!is_device) return fail(FLUTAS_EINVAL, "null pointer");
+  *is_device = 0;
+  if (!ptr) return FLUTAS_OK;
+  hipPointerAttribute_t at;
+  hipError_t e = hipPointerGetAttributes(&at, ptr);
+  if (e != hipSuccess) {
+    // plain (unregistered) host memory: not an error for this query
+    (void)hipGetLastError();
+    return FLUTAS_OK;
+  }
+  *is_device = (at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged) ? 1 : 0;
+  return FLUTAS_OK;
+}
 
 int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
                          const char c_or_f[3], const double dli[3], int nh_d,
@@ -475,8 +489,10 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
                !(env && env[0] == '1');
     if (h->fused) {
       const int gp = fused_group_planes(h->ny, precision == FLUTAS_FP64 ? 8 : 4);
+      // one checkpoint per plane group
       const size_t ngrp = (h->nz + gp - 1) / gp;
-      if (e == hipSuccess) e = hipMalloc((void **)&h->CK, (size_t)h->nx * ngrp * h->ny * sizeof(double));
+      const size_t nck = std::max<size_t>(1, ngrp);
+      if (e == hipSuccess) e = hipMalloc((void **)&h->CK, (size_t)h->nx * nck * h->ny * sizeof(double));
     } else {
       if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
       if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
@@ -552,31 +568,39 @@ int flutas_psolve_solve(flutas_psolve *h, double *p) {
 int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
                           int reps, int max_stages, double *ms, int *nstages) {
   if (!h || !f || !p || !ms || !nstages || reps < 1) return fail(FLUTAS_EINVAL, "bad profile arguments");
+  // The reps solves run back to back on the library stream with one event
+  // after each stage (event 0 before the first); nothing synchronises until
+  // the last event, so the stage times are those of the steady pipeline.
   hipStream_t s = current_stream();
-  std::vector<double> acc;
   h->profiling = true;
+  h->nmark = 0;
   int rc = FLUTAS_OK;
+  std::vector<int> rep_end(reps, 0);
+  if (h->ev.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) rc = fail(FLUTAS_EDEVICE, "hipEventCreate");
+    else h->ev.push_back(e);
+  }
+  if (rc == FLUTAS_OK && hipEventRecord(h->ev[0], s) != hipSuccess)
+    rc = fail(FLUTAS_EDEVICE, "hipEventRecord");
   for (int r = 0; r < reps && rc == FLUTAS_OK; ++r) {
-    h->nmark = 0;
-    if (h->ev.empty()) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventCreate"); break; }
-      h->ev.push_back(e);
-    }
-    if (hipEventRecord(h->ev[0], s) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventRecord"); break; }
     rc = solve_impl(h, f, p);
-    if (rc) break;
-    if (hipEventSynchronize(h->ev[h->nmark]) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventSynchronize"); break; }
-    acc.resize(std::max<size_t>(acc.size(), h->nmark), 0.0);
-    for (int m = 0; m < h->nmark; ++m) {
-      float t = 0.f;
-      if (hipEventElapsedTime(&t, h->ev[m], h->ev[m + 1]) != hipSuccess) { rc = fail(FLUTAS_EDEVICE, "hipEventElapsedTime"); break; }
-      acc[m] += t;
-    }
+    rep_end[r] = h->nmark;
   }
   h->profiling = false;
   if (rc) return rc;
-  const int ns = (int)std::min<size_t>(acc.size(), (size_t)max_stages);
+  FL_HIP(hipEventSynchronize(h->ev[h->nmark]));
+  const int per = rep_end[0];
+  for (int r = 1; r < reps; ++r)
+    if (rep_end[r] - rep_end[r - 1] != per)
+      return fail(FLUTAS_EINVAL, "profile: stage count differs between solves");
+  std::vector<double> acc(per, 0.0);
+  for (int m = 0; m < h->nmark; ++m) {
+    float t = 0.f;
+    FL_HIP(hipEventElapsedTime(&t, h->ev[m], h->ev[m + 1]));
+    acc[m % per] += t;
+  }
+  const int ns = std::min(per, max_stages);
   for (int m = 0; m < ns; ++m) ms[m] = acc[m] / reps;
   *nstages = ns;
   return FLUTAS_OK;

@@ -55,6 +55,9 @@ int flutas_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int flutas_memset_d(void *dst, int value, size_t bytes);
 int flutas_set_stream(void *hip_stream);
 int flutas_synchronize(void);
+/* *is_device = 1 when ptr is a HIP device or managed allocation (usable by
+ * the field kernels as is), 0 for plain host memory. */
+int flutas_pointer_is_device(const void *ptr, int *is_device);
 
 /* ---- Poisson solver --------------------------------------------------- */
 

@@ -63,3 +63,24 @@ def test_invalid_arguments_rejected_without_device():
     with pytest.raises(api.FlutasError) as e:
         api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", precision=16)
     assert e.value.code == api.FLUTAS_EINVAL
+
+
+def test_fortran_dropin_fails_loudly_without_gpu(tmp_path):
+    """The amdflang-built drop-in stops with the library's message when no
+    gfx950 device is present (no silent CPU fallback)."""
+    import subprocess
+    demo = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "projection_demo")
+    if not os.path.exists(demo):
+        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
+    ng, h = (8, 8, 8), 1
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as fh:
+        fh.write(np.array(list(ng) + [h], np.int32).tobytes())
+        fh.write(np.array([1.0, 1.0, 1.0, 0.0, 0.01, 1.0]).tobytes())
+        fh.write(b"PPPPNN")
+        fh.write(np.zeros(3 * 10 ** 3).tobytes())
+    r = subprocess.run([demo, str(fin), str(tmp_path / "out.bin"), "host"],
+                       capture_output=True, text=True, timeout=60)
+    if r.returncode == 0:
+        pytest.skip("a gfx950 device is present")
+    assert "flutas_amd: initsolver failed" in r.stdout + r.stderr

@@ -82,6 +82,7 @@ SOLVE_CASES = [
     ((96, 48, 20), (1.0, 1.0, 1.0), "PPPPDD", 1.2),
     ((128, 192, 18), (1.0, 2.0, 1.0), "DDNNND", 0.0),
     ((256, 64, 34), (1.0, 1.0, 1.0), "NNPPDN", 1.1),
+    ((64, 64, 100), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),  # many plane groups
 ]
 
 

@@ -8,8 +8,8 @@ Neumann z walls, Lx = 2 pi, Ly = pi, tanh stretching 1.5) on a seeded N(0,1)
 right-hand side already resident in HBM.  The line reports ns per grid
 point for the plain FP64 solve (the `value`), the FP32 + FP64-refinement
 solve to a 1e-10 relative residual, per-kernel HBM roofline figures from
-HIP events on the library stream, and the FP64 CPU oracle timed on a
-bounded slab of the same workload on the host cores.
+HIP events on the library stream, and the FP64 CPU oracle timed on the
+same workload on the host cores.
 
 Only one GPU is used: --gpus above 1 exits with a message.
 """
@@ -49,46 +49,46 @@ COPY_BYTES = COPY_N ** 3 * 8
 
 
 def time_steps(fn, steps, warmup, api):
+    """Mean, median and min ms per step from HIP events on the library
+    stream around each step, and the mean wall-clock ms per step."""
     for _ in range(warmup):
         fn()
     api.synchronize()
-    e0, e1 = api.Event(), api.Event()
+    ev = [api.Event() for _ in range(steps + 1)]
     t0 = time.perf_counter()
-    e0.record()
-    for _ in range(steps):
+    ev[0].record()
+    for s in range(steps):
         fn()
-    e1.record()
+        ev[s + 1].record()
     api.synchronize()
     wall = (time.perf_counter() - t0) * 1e3
-    return e0.elapsed_ms(e1) / steps, wall / steps
+    per = [ev[s].elapsed_ms(ev[s + 1]) for s in range(steps)]
+    return {"mean": ev[0].elapsed_ms(ev[steps]) / steps, "median": float(np.median(per)),
+            "min": min(per), "wall": wall / steps}
 
 
-def cpu_baseline(case, nz_sample=128, reps=2):
-    """The C FP64 oracle (solver_cpu restated) on a 512x512xnz_sample slab
-    with the same BCs and stretching; ns per grid point."""
+def cpu_baseline(case, reps=5):
+    """The C FP64 oracle (solver_cpu restated, own FFT, OpenMP) on the same
+    512^3 workload, timed over `reps` solves; ns per grid point (median)."""
     from oracle import pyoracle as po
-    nx, ny, _ = case.ng
-    lx, ly, lz = case.lengths
-    ng = (nx, ny, nz_sample)
-    dzc, dzf, zc, zf = po.initgrid(nz_sample, case.alpha, lz, 1)
-    dli = (nx / lx, ny / ly, nz_sample / lz)
-    s = po.Solver(ng, dli, 1.0 / dzc, 1.0 / dzf, case.cbc)
-    rng = np.random.Generator(np.random.Philox(case.seed))
-    p0 = np.zeros((nx + 2, ny + 2, nz_sample + 2), order="F")
-    p0[1:-1, 1:-1, 1:-1] = rng.standard_normal((nx, ny, nz_sample))
-    best = None
+    from flutas_amd import workloads as wl
+    nx, ny, nz = case.ng
+    dzc, dzf, zc, zf = po.initgrid(nz, case.alpha, case.lengths[2], 1)
+    s = po.Solver(case.ng, case.dli, 1.0 / dzc, 1.0 / dzf, case.cbc)
+    p0 = wl.rhs_normal(case, dzf)
+    times = []
     for _ in range(reps):
         p = p0.copy(order="F")
         t0 = time.perf_counter()
         s.solve(p)
-        dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    npts = nx * ny * nz_sample
-    return {"value": best * 1e9 / npts, "unit": "ns/gridpoint",
+        times.append(time.perf_counter() - t0)
+    npts = nx * ny * nz
+    med = float(np.median(times))
+    return {"value": med * 1e9 / npts, "unit": "ns/gridpoint",
             "cores": int(po.lib().or_max_threads()), "kind": "port",
-            "sample": f"C FP64 oracle (solver_cpu restated, own FFT) on a "
-                      f"{nx}x{ny}x{nz_sample} slab, {case.cbc}, stretch {case.alpha}; "
-                      f"best of {reps}"}
+            "best": min(times) * 1e9 / npts,
+            "sample": f"C FP64 oracle (solver_cpu restated, own FFT, OpenMP) on the same "
+                      f"{nx}x{ny}x{nz} {case.cbc} stretch-{case.alpha} solve; median of {reps}"}
 
 
 def main():
@@ -121,9 +121,11 @@ def main():
 
     # ---- FP64 ---------------------------------------------------------
     plan = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc)
-    ms, wall_ms = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
+    t64 = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
+    ms = t64["mean"]
     relres64 = plan.residual(f, p)
-    log(f"bench: fp64 {ms:.3f} ms/solve (wall {wall_ms:.3f}), relres {relres64:.2e}")
+    log(f"bench: fp64 {ms:.3f} ms/solve (median {t64['median']:.3f}, min {t64['min']:.3f}, "
+        f"wall {t64['wall']:.3f}), relres {relres64:.2e}")
     # per-stage kernel times: HIP events between the stages on the library
     # stream, over profile_reps solves run back to back
     stages = plan.profile(f, p, args.profile_reps)
@@ -149,8 +151,9 @@ def main():
     if not args.no_refine:
         plan32 = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc,
                                  precision=api.FP32_REFINE, tol=1e-10)
-        ms32, _ = time_steps(lambda: plan32.solve_into(f, p), max(3, args.steps // 2),
-                             args.warmup, api)
+        t32 = time_steps(lambda: plan32.solve_into(f, p), max(3, args.steps // 2),
+                         args.warmup, api)
+        ms32 = t32["mean"]
         iters, rel = plan32.stats()
         check = plan32.residual(f, p)
         stages32 = plan32.profile(f, p, max(1, args.profile_reps // 2))
@@ -172,6 +175,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "ms_median": t64["median"],
+        "ms_min": t64["min"],
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f64",

@@ -91,6 +91,61 @@ def cpu_baseline(case, reps=5):
                       f"{nx}x{ny}x{nz} {case.cbc} stretch-{case.alpha} solve; median of {reps}"}
 
 
+def batch_bench(api, wl, reps=5):
+    """The uneven batch (64 problems, nx, ny in {64..256}, nz in {64, 128,
+    256}, three BC patterns, half stretched): aggregate grid points per
+    second of the batched graph (before and after autotuning) and of the
+    same solves run one after another, wall clock per round."""
+    cases = wl.batch_problems()
+    problems, fs, ps, singles = [], [], [], []
+    npts = 0
+    for case in cases:
+        dzc, dzf, dzci, dzfi = wl.z_grid(api, case)
+        f = api.DeviceArray.from_host(wl.rhs_normal(case, dzf))
+        p = api.DeviceArray(f.shape)
+        p.zero()
+        problems.append((case.ng, case.dli, dzci, dzfi, case.cbc))
+        fs.append(f)
+        ps.append(p)
+        npts += case.npoints
+    batch = api.BatchPlan(problems)
+    batch.set_fields(fs, ps)
+
+    def rate(fn):
+        fn()
+        api.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        api.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        return dt * 1e3, npts / dt
+
+    ms_b, pps_b = rate(batch.solve)
+    t0 = time.perf_counter()
+    batch.tune(reps=2)
+    tune_s = time.perf_counter() - t0
+    ms_t, pps_t = rate(batch.solve)
+    th, gh = batch.tuning()
+    batch.destroy()
+    for (ng, dli, dzci, dzfi, cbc) in problems:
+        singles.append(api.PoissonPlan(ng, dli, dzci, dzfi, cbc))
+
+    def sequential():
+        for plan, f, p in zip(singles, fs, ps):
+            plan.solve_into(f, p)
+
+    ms_s, pps_s = rate(sequential)
+    for plan in singles:
+        plan.destroy()
+    return {"problems": len(cases), "grid_points": npts,
+            "batched_ms": ms_b, "batched_points_per_s": pps_b,
+            "tuned_ms": ms_t, "tuned_points_per_s": pps_t, "tune_seconds": tune_s,
+            "tuned_half_tile": sum(th), "tuned_half_group": sum(gh),
+            "sequential_ms": ms_s, "sequential_points_per_s": pps_s,
+            "speedup_vs_sequential": ms_s / ms_t}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +155,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven 64-problem batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy the byte-counter summaries calibrate on")
     args = ap.parse_args()
@@ -197,6 +254,13 @@ def main():
         "fp64_relres": relres64,
         "fp32_refine": refine,
     }
+    if not args.no_batch:
+        log("bench: uneven batch of 64 problems ...")
+        result["batch"] = batch_bench(api, wl)
+        bb = result["batch"]
+        log(f"bench: batch {bb['grid_points'] / 1e6:.1f} M pts: graph {bb['batched_ms']:.2f} ms, "
+            f"tuned {bb['tuned_ms']:.2f} ms, sequential {bb['sequential_ms']:.2f} ms "
+            f"({bb['speedup_vs_sequential']:.2f}x)")
     if args.calibrate_copy:
         del f, p
         a = api.DeviceArray((COPY_N + 2,) * 3)

@@ -240,6 +240,68 @@ class PoissonPlan:
             pass
 
 
+class BatchPlan:
+    """Many independent FP64 problems solved together (flutas_batch_*):
+    one launch per pass over all problems, captured in a hipGraph.
+
+    problems: sequence of (ng, dli, dzci, dzfi, cbc) with nh_d = 1."""
+
+    def __init__(self, problems, nh_d=1):
+        self.problems = [(tuple(int(v) for v in ng), tuple(float(v) for v in dli),
+                          np.ascontiguousarray(dzci, np.float64),
+                          np.ascontiguousarray(dzfi, np.float64), _bc6(cbc))
+                         for ng, dli, dzci, dzfi, cbc in problems]
+        n = len(self.problems)
+        ng = (C.c_int * (3 * n))(*[v for pr in self.problems for v in pr[0]])
+        cbc = "".join(pr[4] for pr in self.problems).encode()
+        dli = (C.c_double * (3 * n))(*[v for pr in self.problems for v in pr[1]])
+        self._keep = [pr[2] for pr in self.problems] + [pr[3] for pr in self.problems]
+        dzci = (_D * n)(*[pr[2].ctypes.data_as(_D) for pr in self.problems])
+        dzfi = (_D * n)(*[pr[3].ctypes.data_as(_D) for pr in self.problems])
+        h = _VP()
+        _check(lib.flutas_batch_create(C.c_int(n), ng, C.c_char_p(cbc), dli, C.c_int(nh_d),
+                                       dzci, dzfi, C.byref(h)))
+        self._h = h.value
+        self._fields = None
+
+    def set_fields(self, fs, ps):
+        """Halo-1 DeviceArrays of every problem: right-hand sides and outputs."""
+        n = len(self.problems)
+        if len(fs) != n or len(ps) != n:
+            raise ValueError("one f and one p per problem")
+        for (ng, *_), f, p in zip(self.problems, fs, ps):
+            for a in (f, p):
+                if a.shape != tuple(v + 2 for v in ng) or a.dtype != np.float64:
+                    raise ValueError("fields must be float64 DeviceArrays of shape ng+2")
+        fv = (_VP * n)(*[f.c for f in fs])
+        pv = (_VP * n)(*[p.c for p in ps])
+        _check(lib.flutas_batch_set_fields(_VP(self._h), fv, pv))
+        self._fields = (list(fs), list(ps))
+
+    def solve(self):
+        _check(lib.flutas_batch_solve(_VP(self._h)))
+
+    def tune(self, reps=3):
+        _check(lib.flutas_batch_tune(_VP(self._h), C.c_int(reps)))
+
+    def tuning(self):
+        n = len(self.problems)
+        th, gh = (C.c_int * n)(), (C.c_int * n)()
+        _check(lib.flutas_batch_tuning(_VP(self._h), th, gh))
+        return list(th), list(gh)
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            lib.flutas_batch_destroy(_VP(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 # ---- reference-named entry points --------------------------------------
 
 def initsolver(n, dims, dims_z, dli, nh_d, dzci, dzfi, cbc, bc, c_or_f,

@@ -226,49 +226,92 @@ __global__ void __launch_bounds__(256)
 // with homogeneous ghost rules (P wrap, N p0 = p1, D p0 = -p1) and the
 // tridiagonal rows a, b, c in z (periodic wrap for PP).
 // One thread walks one (i, j) column through kResidualPlanes planes,
-// keeping the z-neighbours in registers; one partial max per workgroup.
+// keeping the z-neighbours in registers and loading plane k+1 while plane k
+// is evaluated; one partial max per workgroup.
+//
+// The grid is one-dimensional and its workgroups are renumbered so that
+// consecutive rows j of one plane chunk run on the same XCD (the hardware
+// deals workgroups to the 8 XCDs round robin): the rows j-1 and j+1 that a
+// row reads as neighbours are then the rows its XCD's L2 is already
+// streaming.
 constexpr int kResidualPlanes = 32;
+constexpr int kXcds = 8;
+
+struct BlockIdx3 {
+  int ib, j, kc;
+};
+__device__ __forceinline__ BlockIdx3 residual_block(int nbx, int ny) {
+  const unsigned w = blockIdx.x, nb = gridDim.x;
+  const unsigned per = nb / kXcds;
+  const unsigned L = (w < per * kXcds) ? (w % kXcds) * per + w / kXcds : w;
+  BlockIdx3 b;
+  b.ib = (int)(L % (unsigned)nbx);
+  const unsigned r = L / (unsigned)nbx;
+  b.j = (int)(r % (unsigned)ny);
+  b.kc = (int)(r / (unsigned)ny);
+  return b;
+}
+
+// The values of one plane that the residual of cell (i, j, k) reads.
+struct Plane7 {
+  double f, c, xm, xp, ym, yp;
+};
 
 template <typename Tr>
 __global__ void __launch_bounds__(256)
     k_residual(OperatorDesc op, const double *__restrict__ f,
                const double *__restrict__ p, Tr *__restrict__ r,
-               double *__restrict__ part) {
+               double *__restrict__ part, int nbx) {
 #pragma clang fp contract(off)
   __shared__ double smax[256];
   const int nx = op.nx, ny = op.ny, nz = op.nz;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  const int k0 = blockIdx.z * kResidualPlanes;
+  const BlockIdx3 bi = residual_block(nbx, ny);
+  const int i = bi.ib * blockDim.x + threadIdx.x;
+  const int j = bi.j;
+  const int k0 = bi.kc * kResidualPlanes;
   const int k1 = min(nz, k0 + kResidualPlanes);
+  const bool zper = op.cbc[4] == 'P';
   double am = 0.0;
   if (i < nx) {
     const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
-    long long c = (i + 1) + sy * (j + 1) + sz * (k0 + 1);
-    double pc = p[c];
-    double pzm = (k0 > 0) ? p[c - sz] : ((op.cbc[4] == 'P') ? p[c + sz * (nz - 1)] : 0.0);
-    for (int k = k0; k < k1; ++k, c += sz) {
-      double pm, pp;
-      if (i > 0) pm = p[c - 1];
-      else pm = (op.cbc[0] == 'P') ? p[c + (nx - 1)] : (op.cbc[0] == 'N') ? pc : -pc;
-      if (i < nx - 1) pp = p[c + 1];
-      else pp = (op.cbc[1] == 'P') ? p[c - (nx - 1)] : (op.cbc[1] == 'N') ? pc : -pc;
-      const double lx = (pp - 2.0 * pc + pm) * op.dx2;
-      if (j > 0) pm = p[c - sy];
-      else pm = (op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : (op.cbc[2] == 'N') ? pc : -pc;
-      if (j < ny - 1) pp = p[c + sy];
-      else pp = (op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : (op.cbc[3] == 'N') ? pc : -pc;
-      const double ly = (pp - 2.0 * pc + pm) * op.dy2;
-      const double pzp = (k < nz - 1) ? p[c + sz] : ((op.cbc[5] == 'P') ? p[c - sz * (nz - 1)] : 0.0);
+    const long long c0 = (i + 1) + sy * (j + 1);  // plane-0 offset minus sz
+    // x and y neighbours with the homogeneous ghost rules
+    auto gather = [&](int k) {
+      const long long c = c0 + sz * (k + 1);
+      Plane7 v;
+      v.f = f[c];
+      v.c = p[c];
+      v.xm = (i > 0) ? p[c - 1] : ((op.cbc[0] == 'P') ? p[c + (nx - 1)] : 0.0);
+      v.xp = (i < nx - 1) ? p[c + 1] : ((op.cbc[1] == 'P') ? p[c - (nx - 1)] : 0.0);
+      v.ym = (j > 0) ? p[c - sy] : ((op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : 0.0);
+      v.yp = (j < ny - 1) ? p[c + sy] : ((op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : 0.0);
+      return v;
+    };
+    auto ghost = [&](double nb, double pc, char bc, bool inside) {
+      if (inside || bc == 'P') return nb;
+      return bc == 'N' ? pc : -pc;
+    };
+    double pzm = (k0 > 0) ? p[c0 + sz * k0] : (zper ? p[c0 + sz * nz] : 0.0);
+    Plane7 cur = gather(k0);
+    for (int k = k0; k < k1; ++k) {
+      // plane k+1 (the z neighbour, and the next plane of the walk)
+      const int kn = (k + 1 < nz) ? k + 1 : (zper ? 0 : k);
+      const Plane7 nxt = gather(kn);
+      const double pc = cur.c;
+      const double lx = (ghost(cur.xp, pc, op.cbc[1], i < nx - 1) - 2.0 * pc +
+                         ghost(cur.xm, pc, op.cbc[0], i > 0)) * op.dx2;
+      const double ly = (ghost(cur.yp, pc, op.cbc[3], j < ny - 1) - 2.0 * pc +
+                         ghost(cur.ym, pc, op.cbc[2], j > 0)) * op.dy2;
+      const double pzp = (k < nz - 1 || zper) ? nxt.c : 0.0;
       double lz = op.b[k] * pc;
-      if (k > 0 || op.cbc[4] == 'P') lz += op.a[k] * pzm;
-      if (k < nz - 1 || op.cbc[5] == 'P') lz += op.c[k] * pzp;
-      const double res = f[c] - ((lx + ly) + lz);
+      if (k > 0 || zper) lz += op.a[k] * pzm;
+      if (k < nz - 1 || zper) lz += op.c[k] * pzp;
+      const double res = cur.f - ((lx + ly) + lz);
       if (r) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
       const double ar = fabs(res);
       if (ar > am || res != res) am = (res != res) ? res : ar;
       pzm = pc;
-      pc = pzp;
+      cur = nxt;
     }
   }
   smax[threadIdx.x] = am;
@@ -280,8 +323,7 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    part[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = smax[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = smax[0];
 }
 
 __global__ void __launch_bounds__(256)
@@ -364,14 +406,20 @@ template <typename Tr>
 int launch_residual(const OperatorDesc &op, const double *f, const double *p,
                     Tr *r, double *partial, int *nparts, hipStream_t s) {
   const int bx = 256;
-  dim3 grid((op.nx + bx - 1) / bx, op.ny, (op.nz + kResidualPlanes - 1) / kResidualPlanes);
-  hipLaunchKernelGGL(k_residual<Tr>, grid, dim3(bx), 0, s, op, f, p, r, partial);
+  const int nbx = (op.nx + bx - 1) / bx;
+  const long long nb = (long long)nbx * op.ny * ((op.nz + kResidualPlanes - 1) / kResidualPlanes);
+  if (nb > 0x7fffffffLL) return fail(FLUTAS_EINVAL, "residual grid too large");
+  hipLaunchKernelGGL(k_residual<Tr>, dim3((unsigned)nb), dim3(bx), 0, s, op, f, p, r, partial, nbx);
   FL_HIP(hipGetLastError());
-  *nparts = (int)(grid.x * grid.y * grid.z);
+  *nparts = (int)nb;
   return FLUTAS_OK;
 }
 template int launch_residual<float>(const OperatorDesc &, const double *, const double *, float *, double *, int *, hipStream_t);
 template int launch_residual<double>(const OperatorDesc &, const double *, const double *, double *, double *, int *, hipStream_t);
+
+size_t residual_blocks(int nx, int ny, int nz) {
+  return (size_t)((nx + 255) / 256) * ny * ((nz + kResidualPlanes - 1) / kResidualPlanes);
+}
 
 int launch_absmax(const double *f, int nx, int ny, int nz, double *partial,
                   int *nparts, hipStream_t s) {

@@ -1,0 +1,654 @@
+// Device bodies of the fused four-pass pressure solve (the hot path),
+// shared by the single-problem kernels (fused.hip) and the batched kernels
+// (batch.hip).
+//
+// Work layout Wt[k][i][j] (j fastest): y-lines are contiguous, and an
+// x-tile of P1/P4 (fixed k) touches a single 2-D plane.
+//
+//   P1 x_fwd_tile   p (halo layout, x-lines) -> x r2r -> Wt (transposed
+//                   store)
+//   P2 yz_fwd_slab  per x-slab i, k ascending: y r2r of each plane, then
+//                   the Thomas forward elimination step for all (i, j)
+//                   columns
+//   P3 zy_bwd_slab  per x-slab i, k descending: Thomas back substitution,
+//                   then the inverse y r2r of each plane
+//   P4 x_bwd_tile   Wt (transposed load) -> inverse x r2r * normfft -> p
+//
+// Each pass reads and writes the field once.  The elimination factors d(k)
+// are not stored: P2 writes one checkpoint d per column every GP planes and
+// P3 recomputes the factors of each plane group from it with the same
+// operations, so the pivots remain the reference's bit for bit
+// (src/solver_cpu.f90:187-223, contraction off).
+//
+// Memory-level parallelism: every thread issues all global loads of a tile
+// before touching LDS, and the slab kernels P2/P3 load the next plane group
+// into registers while the current group is transformed and eliminated.
+#pragma once
+
+#include "tile_fft.hpp"
+
+namespace flutas {
+
+using tile::cx;
+
+// LDS line stride: two extra elements per line so that the transposed
+// accesses of P1/P4 (same element of consecutive lines) spread over banks.
+template <typename T, int n>
+constexpr int line_stride() {
+  return n + (sizeof(T) == 8 ? 2 : 4);
+}
+
+// Lines per tile: the largest of 16, 8, 4, 2, 1 whose LDS tile fits
+// kTileBytes (two tiles per CU fit the 160 KiB of gfx950 LDS).
+constexpr int kTileBytes = 72 * 1024;
+template <typename T, int NC>
+constexpr int tile_lines() {
+  constexpr int bytes = line_stride<T, 2 * NC>() * (int)sizeof(T);
+  constexpr int l = kTileBytes / bytes;
+  return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
+}
+
+// Twiddle tables staged in LDS once per workgroup: exp(-2 pi i m/N) (N),
+// exp(-2 pi i k/n) (N+1) and exp(-i pi k/2n) (N+1, the entries the real
+// split and the cosine/sine phases use).  Keeping them out of global memory
+// leaves the vector-memory counter to the field loads, so a prefetch is not
+// drained by the first twiddle read.
+template <typename T, int NC>
+struct Tw {
+  static constexpr int ntw = 2 * tile::Plan<NC>::tw_total();
+  static constexpr int nth = 2 * (NC + 1), ntq = 2 * (NC + 1);
+  static constexpr int total = ntw + nth + ntq;
+  const T *tw, *th, *tq;
+};
+template <typename T, int NC, int NT>
+__device__ __forceinline__ Tw<T, NC> stage_twiddles(T *s, const T *__restrict__ tw,
+                                                    const T *__restrict__ twh,
+                                                    const T *__restrict__ twq) {
+  using W = Tw<T, NC>;
+  // fixed trip counts with clamped indices: the loads are issued back to
+  // back without branches, then stored to LDS
+  constexpr int Q1 = (W::ntw + NT - 1) / NT, Q2 = (W::nth + NT - 1) / NT;
+  T v1[Q1], v2[Q2], v3[Q2];
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) v1[q] = tw[min((int)threadIdx.x + q * NT, W::ntw - 1)];
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) {
+    const int t = min((int)threadIdx.x + q * NT, W::nth - 1);
+    v2[q] = twh[t];
+    v3[q] = twq[t];
+  }
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < W::ntw) s[t] = v1[q];
+  }
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < W::nth) {
+      s[W::ntw + t] = v2[q];
+      s[W::ntw + W::nth + t] = v3[q];
+    }
+  }
+  return Tw<T, NC>{s, s + W::ntw, s + W::ntw + W::nth};
+}
+
+// max(m, |v|) with NaN propagation (a NaN in either stays NaN)
+__device__ __forceinline__ double nan_max(double m, double v) {
+  const double av = fabs(v);
+  return (v != v) ? v : ((av > m) ? av : m);
+}
+
+// Workgroup max of one value per thread (NaN propagating), stored by
+// thread 0.  Contains a barrier; every thread must call it.
+template <int NT>
+__device__ __forceinline__ void block_max_store(double am, double *dst) {
+  __shared__ double wmax[NT / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) am = nan_max(am, __shfl_xor(am, off));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = am;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) m = nan_max(m, wmax[w]);
+    *dst = m;
+  }
+}
+
+// Minimum waves per SIMD of a P2/P3 workgroup size: two 512-thread slab
+// workgroups per CU (4 waves per SIMD, at most 128 VGPRs each), since the
+// grid has only one workgroup per x-slab.
+constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
+
+// ---------------------------------------------------------------- P1 -----
+// One workgroup: TJ consecutive x-lines (j0..j0+TJ-1) at one k.
+// in: element (i, j, k) at in[j*is1 + k*is2 + i] (i, j, k 0-based).
+// Wt: element (i, j, k) at Wt[(k*nx + i)*ny + j].
+// The tile body (also used by the batched kernels): sm holds TJ lines of
+// line_stride<T, nx>() elements, stw the staged twiddles; absmax_slot, when
+// not null, receives the tile's max |in|.
+template <typename T, typename Tin, int NC, int TJ, int NT>
+__device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long is1,
+                                           long long is2, T *__restrict__ Wt, int ny,
+                                           int kind, const T *__restrict__ tw,
+                                           const T *__restrict__ twh,
+                                           const T *__restrict__ twq, double *absmax_slot,
+                                           T *sm, T *stw, int j0, int k) {
+  constexpr int n = 2 * NC;  // nx
+  constexpr int LS = line_stride<T, n>();
+  constexpr int LD = (TJ * n + NT - 1) / NT;
+  const int nl = min(TJ, ny - j0);
+  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  Tin r[LD];
+  // branch-free loads: lines past ny (partial tile) re-read line ny-1
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
+    const int l = t / n, e = t - (t / n) * n;
+    r[q] = in[(long long)min(j0 + l, ny - 1) * is1 + (long long)k * is2 + e];
+  }
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t >= TJ * n) continue;
+    const int l = t / n, e = t - (t / n) * n;
+    T x = (T)r[q];
+    int pos = e;
+    if (trig) {
+      pos = tile::mk_pos(e, n);
+      if (sine && (e & 1)) x = -x;
+    }
+    sm[l * LS + pos] = x;
+  }
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  if (absmax_slot) {
+    // max |in| over the tile (the clamped loads only repeat tile entries),
+    // NaN propagating
+    double am = 0.0;
+#pragma unroll
+    for (int q = 0; q < LD; ++q) am = nan_max(am, (double)r[q]);
+    block_max_store<NT>(am, absmax_slot);
+  }
+  __syncthreads();
+  tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
+  tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  // transposed store
+  T *plane = Wt + (long long)k * n * ny;
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = threadIdx.x + q * NT;
+    const int l = t % TJ, i = t / TJ;
+    if (t < TJ * n && l < nl) plane[(long long)i * ny + j0 + l] = sm[l * LS + i];
+  }
+}
+
+// ---------------------------------------------------------------- P4 -----
+// Inverse of P1: out(i, j, k) = scale * r2r^-1 (x) (+ out when ACC).
+template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
+__device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, Tout *__restrict__ out,
+                                           long long os1, long long os2, int ny, int kind,
+                                           T scale, const T *__restrict__ tw,
+                                           const T *__restrict__ twh,
+                                           const T *__restrict__ twq, T *sm, T *stw, int j0,
+                                           int k) {
+  constexpr int n = 2 * NC;
+  constexpr int LS = line_stride<T, n>();
+  constexpr int LD = (TJ * n + NT - 1) / NT;
+  const int nl = min(TJ, ny - j0);
+  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  const T *plane = Wt + (long long)k * n * ny;
+  T r[LD];
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
+    const int l = t % TJ, i = t / TJ;
+    r[q] = plane[(long long)i * ny + min(j0 + l, ny - 1)];
+  }
+  // the previous values of an accumulated output are loaded with the tile
+  // so that their latency overlaps the transform
+  Tout prev[ACC ? LD : 1];
+  if constexpr (ACC) {
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
+      const int l = t / n, e = t - (t / n) * n;
+      prev[q] = out[(long long)min(j0 + l, ny - 1) * os1 + (long long)k * os2 + e];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (t < TJ * n) sm[(t % TJ) * LS + t / TJ] = r[q];
+  }
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  __syncthreads();
+  tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  tile::cfft<T, +1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
+#pragma unroll
+  for (int q = 0; q < LD; ++q) {
+    const int t = threadIdx.x + q * NT;
+    const int l = t / n, e = t - (t / n) * n;
+    if (t >= TJ * n || l >= nl) continue;
+    int pos = e;
+    T sg = T(1);
+    if (trig) {
+      pos = tile::mk_pos(e, n);
+      if (sine && (e & 1)) sg = T(-1);
+    }
+    const T v = sm[l * LS + pos] * (scale * sg);
+    Tout *o = out + (long long)(j0 + l) * os1 + (long long)k * os2 + e;
+    if constexpr (ACC) *o = (Tout)((double)prev[q] + (double)v);
+    else *o = (Tout)v;
+  }
+}
+
+// ---------------------------------------------------------------- P2 -----
+// One workgroup per x-slab i; planes are processed GP at a time.
+// Checkpoints: ck[(i*nck + g)*ny + j] = d of the last plane of group g
+// (nck = number of groups); P3 restarts the recurrence from them.  (One
+// checkpoint per group: replaying a group of divides in P3 costs more
+// latency than the 1/GP extra traffic.)
+template <int GP>
+__host__ __device__ constexpr int checkpoints(int nz) {
+  return (nz + GP - 1) / GP;
+}
+
+// The slab body (also used by the batched kernels): sm holds 2*GP lines
+// of line_stride<T, ny>() elements, stw the staged twiddles.
+template <typename T, int NC, int GP, int NT, bool PER>
+__device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restrict__ ck,
+                                            int nx, int nz, int kind,
+                                            const double *__restrict__ lamx,
+                                            const double *__restrict__ lamy,
+                                            const double *__restrict__ a,
+                                            const double *__restrict__ b,
+                                            const double *__restrict__ c, int pin,
+                                            const T *__restrict__ tw,
+                                            const T *__restrict__ twh,
+                                            const T *__restrict__ twq, T *sm, T *stw,
+                                            int i) {
+#pragma clang fp contract(off)
+  constexpr int n = 2 * NC;  // ny
+  constexpr int LS = line_stride<T, n>();
+  constexpr int CPT = (n + NT - 1) / NT;  // columns per thread
+  constexpr int LD = (GP * n + NT - 1) / NT;
+  T *const A = sm, *const B = sm + GP * LS;
+  // the complex FFT ends in A after an even number of passes
+  constexpr bool fft_in_b = (tile::Plan<NC>::nst % 2) == 1;
+  T *const Z = fft_in_b ? B : A;  // complex FFT result
+  T *const Y = fft_in_b ? A : B;  // spectral lines after the real split
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  const int ngrp = (nz + GP - 1) / GP;
+  const int nck = checkpoints<GP>(nz);
+  const long long pstride = (long long)nx * n;  // plane stride
+  T *line0 = Wt + (long long)i * n;             // plane 0 line of slab i
+  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  const double lx = lamx[i];
+  double lam[CPT], dprev[CPT], pprev[CPT];
+  // periodic z only: second right-hand side, sums giving p1(0) and p2(0)
+  double q2prev[CPT], s1[CPT], s2[CPT], gk[CPT];
+#pragma unroll
+  for (int u = 0; u < CPT; ++u) {
+    const int j = threadIdx.x + u * NT;
+    lam[u] = (j < n) ? lx + lamy[j] : 0.0;
+    dprev[u] = 0.0;
+    pprev[u] = 0.0;
+    q2prev[u] = 0.0;
+    s1[u] = 0.0;
+    s2[u] = 0.0;
+    gk[u] = 1.0;
+  }
+  T r[LD];
+  // branch-free prefetch: planes past nz (last group) re-read plane nz-1
+  auto load = [&](int g) {
+    const int k0 = g * GP;
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int t = min((int)threadIdx.x + q * NT, GP * n - 1);
+      const int l = t / n, e = t - (t / n) * n;
+      r[q] = line0[(long long)min(k0 + l, nz - 1) * pstride + e];
+    }
+  };
+  // Results of a group (and its checkpoint) stay in registers and are
+  // stored in the next iteration, before the prefetch of the group after:
+  // the wait for that prefetch then also covers stores that have had a
+  // whole group of work to drain, instead of stores issued just before it.
+  T out[GP][CPT];
+  double ckout[CPT], ck2out[CPT];
+  auto store = [&](int g) {
+    const int k0 = g * GP;
+    const int np = min(GP, nz - k0);
+#pragma unroll
+    for (int l = 0; l < GP; ++l) {
+      if (l >= np) continue;
+      T *line = line0 + (long long)(k0 + l) * pstride;
+#pragma unroll
+      for (int u = 0; u < CPT; ++u) {
+        const int j = threadIdx.x + u * NT;
+        if (j < n) line[j] = out[l][u];
+      }
+    }
+    // checkpoint of group g: ck[(i*nck + g)*ny + j] (periodic z: the
+    // second right-hand side's p' follows at offset nx*nck*ny)
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int j = threadIdx.x + u * NT;
+      const long long o = ((long long)i * nck + g) * n + j;
+      if (j < n) {
+        ck[o] = ckout[u];
+        if constexpr (PER) ck[o + (long long)nx * nck * n] = ck2out[u];
+      }
+    }
+  };
+  load(0);
+  for (int g = 0; g < ngrp; ++g) {
+    const int k0 = g * GP;
+    const int np = min(GP, nz - k0);
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int t = threadIdx.x + q * NT;
+      if (t >= GP * n) continue;
+      const int l = t / n, e = t - (t / n) * n;
+      T x = r[q];
+      int pos = e;
+      if (trig) {
+        pos = tile::mk_pos(e, n);
+        if (sine && (e & 1)) x = -x;
+      }
+      A[l * LS + pos] = x;
+    }
+    __syncthreads();
+    if (g > 0) store(g - 1);
+    if (g + 1 < ngrp) load(g + 1);
+    tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
+                                            reinterpret_cast<cx<T> *>(B), W.tw);
+    tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
+    // forward elimination, plane by plane (dgtsv_homebrewed order)
+#pragma unroll
+    for (int l = 0; l < GP; ++l) {
+      if (l >= np) continue;
+      const int k = k0 + l;
+      const double ak = a[k], bk = b[k], cc = c[k];
+#pragma unroll
+      for (int u = 0; u < CPT; ++u) {
+        const int j = threadIdx.x + u * NT;
+        if (j >= n) continue;
+        const double y = (double)Y[l * LS + j];
+        double pv;
+        if constexpr (!PER) {
+          if (k == 0) {
+            const double z = 1.0 / (bk + lam[u]);
+            dprev[u] = cc * z;
+            pv = y * z;
+          } else if (k < nz - 1) {
+            const double bb = bk + lam[u];
+            const double z = 1.0 / (bb - ak * dprev[u]);
+            dprev[u] = cc * z;
+            pv = (y - ak * pprev[u]) * z;
+          } else {
+            const double z = (bk + lam[u]) - ak * dprev[u];
+            if (z != 0.0 && !(pin && lam[u] == 0.0))
+              pv = (y - ak * pprev[u]) / z;
+            else
+              pv = 0.0;
+          }
+        } else {
+          // cyclic system (src/solver_cpu.f90:147-185): the first m = nz-1
+          // rows are eliminated for the field (p1) and for
+          // r2 = (-a(0), 0, ..., 0, -c(m-1)) (p2); p1(0) and p2(0) are the
+          // sums of g(k) p'(k), g(k) = prod_{k'<k} (-d(k')), so the last
+          // plane is closed here and P3 needs no second sweep.
+          const int m = nz - 1;
+          if (k < m) {
+            const double r2 = (k == m - 1) ? -c[m - 1] : ((k == 0) ? -a[0] : 0.0);
+            double q2;
+            if (k == 0) {
+              const double z = 1.0 / (bk + lam[u]);
+              dprev[u] = cc * z;
+              pv = y * z;
+              q2 = r2 * z;
+            } else if (k < m - 1) {
+              const double z = 1.0 / ((bk + lam[u]) - ak * dprev[u]);
+              dprev[u] = cc * z;
+              pv = (y - ak * pprev[u]) * z;
+              q2 = (r2 - ak * q2prev[u]) * z;
+            } else {
+              const double z = (bk + lam[u]) - ak * dprev[u];
+              if (z != 0.0) {
+                pv = (y - ak * pprev[u]) / z;
+                q2 = (r2 - ak * q2prev[u]) / z;
+              } else {
+                pv = 0.0;
+                q2 = 0.0;
+              }
+            }
+            s1[u] = s1[u] + gk[u] * pv;
+            s2[u] = s2[u] + gk[u] * q2;
+            gk[u] = -(dprev[u] * gk[u]);
+            q2prev[u] = q2;
+          } else if (pin && lam[u] == 0.0) {
+            pv = 0.0;
+          } else {
+            pv = (y - c[m] * s1[u] - a[m] * pprev[u]) /
+                 ((bk + lam[u]) + c[m] * s2[u] + a[m] * q2prev[u]);
+          }
+        }
+        pprev[u] = pv;
+        out[l][u] = (T)pv;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      ckout[u] = dprev[u];
+      ck2out[u] = q2prev[u];
+    }
+    __syncthreads();
+  }
+  store(ngrp - 1);
+}
+
+// ---------------------------------------------------------------- P3 -----
+template <typename T, int NC, int GP, int NT, bool PER>
+__device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__restrict__ ck,
+                                            int nx, int nz, int kind,
+                                            const double *__restrict__ lamx,
+                                            const double *__restrict__ lamy,
+                                            const double *__restrict__ a,
+                                            const double *__restrict__ b,
+                                            const double *__restrict__ c,
+                                            const T *__restrict__ tw,
+                                            const T *__restrict__ twh,
+                                            const T *__restrict__ twq, T *sm, T *stw,
+                                            int i) {
+#pragma clang fp contract(off)
+  constexpr int n = 2 * NC;
+  constexpr int LS = line_stride<T, n>();
+  constexpr int CPT = (n + NT - 1) / NT;
+  T *const A = sm, *const B = sm + GP * LS;
+  // pre_inv A -> B, then the FFT passes start in B
+  constexpr bool fft_in_a = (tile::Plan<NC>::nst % 2) == 1;
+  T *const R = fft_in_a ? A : B;  // inverse FFT result
+  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  const int ngrp = (nz + GP - 1) / GP;
+  const int nck = checkpoints<GP>(nz);
+  const long long pstride = (long long)nx * n;
+  T *line0 = Wt + (long long)i * n;
+  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  const double lx = lamx[i];
+  double lam[CPT], pnext[CPT];
+  // periodic z only: p2 of the next plane, the closed last plane
+  double pnext2[CPT], plast[CPT];
+#pragma unroll
+  for (int u = 0; u < CPT; ++u) {
+    const int j = threadIdx.x + u * NT;
+    lam[u] = (j < n) ? lx + lamy[j] : 0.0;
+    pnext[u] = 0.0;
+    pnext2[u] = 0.0;
+    plast[u] = 0.0;
+  }
+  // p' of a plane group in column-owner layout (thread owns columns j)
+  T pv[GP][CPT];
+  double dck[CPT], q2ck[CPT];
+  auto load = [&](int g) {
+    const int k0 = g * GP;
+    // branch-free: out-of-range planes/columns re-read valid entries
+#pragma unroll
+    for (int l = 0; l < GP; ++l)
+#pragma unroll
+      for (int u = 0; u < CPT; ++u) {
+        const int j = min((int)threadIdx.x + u * NT, n - 1);
+        pv[l][u] = line0[(long long)min(k0 + l, nz - 1) * pstride + j];
+      }
+    // the recurrence restarts at the end of group g - 1 (unused, but read
+    // in bounds, for g = 0)
+    const int c0 = max(g - 1, 0);
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int j = min((int)threadIdx.x + u * NT, n - 1);
+      const long long o = ((long long)i * nck + c0) * n + j;
+      dck[u] = ck[o];
+      if constexpr (PER) q2ck[u] = ck[o + (long long)nx * nck * n];
+    }
+  };
+  constexpr int LD = (GP * n + NT - 1) / NT;
+  T outv[LD];
+  auto store = [&](int g) {
+    const int k0 = g * GP;
+    const int np = min(GP, nz - k0);
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int t = threadIdx.x + q * NT;
+      const int l = t / n, e = t - (t / n) * n;
+      if (t < GP * n && l < np) line0[(long long)(k0 + l) * pstride + e] = outv[q];
+    }
+  };
+  load(ngrp - 1);
+  for (int g = ngrp - 1; g >= 0; --g) {
+    const int k0 = g * GP;
+    const int np = min(GP, nz - k0);
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int j = threadIdx.x + u * NT;
+      if (j >= n) continue;
+      // elimination factors of this group, recomputed from the checkpoint
+      double d[GP], q2[GP];
+      double dp = dck[u];
+      if constexpr (!PER) {
+#pragma unroll
+        for (int l = 0; l < GP; ++l) {
+          const int k = k0 + l;
+          if (l < np && k < nz - 1) {
+            if (k == 0) dp = c[0] * (1.0 / (b[0] + lam[u]));
+            else dp = c[k] * (1.0 / ((b[k] + lam[u]) - a[k] * dp));
+          }
+          d[l] = dp;
+        }
+        double pn = pnext[u];
+#pragma unroll
+        for (int l = GP - 1; l >= 0; --l) {
+          if (l >= np) continue;
+          const int k = k0 + l;
+          const double p = (k == nz - 1) ? (double)pv[l][u] : (double)pv[l][u] - d[l] * pn;
+          pn = p;
+          A[l * LS + j] = (T)p;
+        }
+        pnext[u] = pn;
+      } else {
+        // cyclic system: p = p1 + p2 * p(last) below the last plane, with
+        // p1 from the stored p' and p2 from the recomputed r2 elimination
+        const int m = nz - 1;
+        double qp = q2ck[u];
+#pragma unroll
+        for (int l = 0; l < GP; ++l) {
+          const int k = k0 + l;
+          if (l < np && k < m) {
+            const double r2 = (k == m - 1) ? -c[m - 1] : ((k == 0) ? -a[0] : 0.0);
+            if (k == 0) {
+              const double z = 1.0 / (b[0] + lam[u]);
+              dp = c[0] * z;
+              qp = r2 * z;
+            } else if (k < m - 1) {
+              const double z = 1.0 / ((b[k] + lam[u]) - a[k] * dp);
+              dp = c[k] * z;
+              qp = (r2 - a[k] * qp) * z;
+            } else {
+              const double z = (b[k] + lam[u]) - a[k] * dp;
+              qp = (z != 0.0) ? (r2 - a[k] * qp) / z : 0.0;
+            }
+          }
+          if (k == nz - 1) plast[u] = (double)pv[l][u];
+          d[l] = dp;
+          q2[l] = qp;
+        }
+        double pn = pnext[u], pn2 = pnext2[u];
+#pragma unroll
+        for (int l = GP - 1; l >= 0; --l) {
+          if (l >= np) continue;
+          const int k = k0 + l;
+          double p;
+          if (k == nz - 1) {
+            p = (double)pv[l][u];
+          } else {
+            double p1, p2;
+            if (k == m - 1) {
+              p1 = (double)pv[l][u];
+              p2 = q2[l];
+            } else {
+              p1 = (double)pv[l][u] - d[l] * pn;
+              p2 = q2[l] - d[l] * pn2;
+            }
+            pn = p1;
+            pn2 = p2;
+            p = p1 + p2 * plast[u];
+          }
+          A[l * LS + j] = (T)p;
+        }
+        pnext[u] = pn;
+        pnext2[u] = pn2;
+      }
+    }
+    __syncthreads();
+    if (g < ngrp - 1) store(g + 1);
+    if (g > 0) load(g - 1);
+    tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
+    tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
+                                            reinterpret_cast<cx<T> *>(A), W.tw);
+    // the group's result goes to registers; it is stored in the next
+    // iteration, before the prefetch of the group after
+#pragma unroll
+    for (int q = 0; q < LD; ++q) {
+      const int t = min((int)threadIdx.x + q * NT, GP * n - 1);
+      const int l = t / n, e = t - (t / n) * n;
+      int pos = e;
+      T sg = T(1);
+      if (trig) {
+        pos = tile::mk_pos(e, n);
+        if (sine && (e & 1)) sg = T(-1);
+      }
+      outv[q] = R[l * LS + pos] * sg;
+    }
+    __syncthreads();
+  }
+  store(0);
+}
+
+// P2/P3 plane groups: the two ping-pong tiles of a group stay within
+// kSlabBytes so that two slab workgroups fit one CU.
+constexpr int kSlabBytes = 72 * 1024;
+template <typename T, int NC>
+constexpr int group_planes() {
+  constexpr int bytes = 2 * line_stride<T, 2 * NC>() * (int)sizeof(T);
+  constexpr int l = kSlabBytes / bytes;
+  return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
+}
+
+// Threads of a P2/P3 slab workgroup (y-lines of 512 points or more use
+// kSlabThreadsLong).
+constexpr int kSlabThreadsLong = 256;
+template <int NC>
+constexpr int slab_threads() {
+  return 2 * NC >= 512 ? kSlabThreadsLong : 256;
+}
+
+}  // namespace flutas

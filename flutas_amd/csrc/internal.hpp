@@ -97,15 +97,18 @@ int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
                            const double *b, const double *c, int nx, int ny,
                            int nz, int pin, hipStream_t s);
 
-// ---- fused four-pass pipeline (non-periodic z) ---------------------------
-// Work layout Wt[i][k][j]; see fused.hip.
+// ---- fused four-pass pipeline --------------------------------------------
+// Work layout Wt[k][i][j]; see fused.hip.
 bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
 // Uploads the per-pass twiddle tables of the compiled plan of tab.n.
 int build_fused_twiddles(LineTables &tab);
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
-                 long long is2, T *Wt, int ny, int nz, hipStream_t s);
+                 long long is2, T *Wt, int ny, int nz, double *absmax_part,
+                 hipStream_t s);
+// workgroups of launch_x_fwd (one max|in| partial each when absmax_part)
+int x_fwd_blocks(const LineTables &tx, int elem_bytes, int ny, int nz);
 template <typename T, typename Tout>
 int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
                  long long os1, long long os2, int ny, int nz, T scale,
@@ -114,12 +117,48 @@ template <typename T>
 int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, double *ck, int nx,
                   int nz, const double *lamx, const double *lamy,
                   const double *a, const double *b, const double *c, int pin,
-                  hipStream_t s);
+                  int periodic, hipStream_t s);
 template <typename T>
 int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
                   int nx, int nz, const double *lamx, const double *lamy,
                   const double *a, const double *b, const double *c,
-                  hipStream_t s);
+                  int periodic, hipStream_t s);
+
+// ---- batched fused pipeline (many independent FP64 problems) ------------
+// One launch per stage covers every problem of a batch; a workgroup reads
+// its (problem, tile) from the item table and runs the body compiled for
+// that problem's line lengths.
+struct BatchProblem {
+  int nx, ny, nz;
+  int ncx, ncy;                    // nx / 2, ny / 2 (compiled plan selector)
+  int kx_f, kx_b, ky_f, ky_b;      // r2r kinds of the four passes
+  int per, pin;                    // periodic z; singular column pinned
+  int tj_half, gp_half;            // tuned variants: half tile / half group
+  double scale;                    // normfft
+  long long hx, hxy;               // strides of the halo-1 fields
+  const double *f;                 // right-hand side (interior origin)
+  double *p;                       // solution (interior origin)
+  double *W, *ck;                  // work field, z checkpoints
+  const double *lamx, *lamy, *a, *b, *c;
+  const double *twx, *twhx, *twqx, *twy, *twhy, *twqy;
+};
+struct BatchItem {
+  int prob, u, v;  // x passes: (tile, plane); slab passes: (slab, 0)
+};
+enum { BATCH_X_FWD = 0, BATCH_YZ_FWD = 1, BATCH_ZY_BWD = 2, BATCH_X_BWD = 3 };
+// line lengths the batched kernels are compiled for
+bool batch_supported(int n);
+// lines per x tile and planes per slab group of a length (FP64), full or
+// halved variant
+int batch_tile_lines(int nx, int half);
+int batch_group_planes(int ny, int half);
+// dynamic LDS bytes of a stage for line length n (the largest variant)
+size_t batch_lds_bytes(int stage, int n);
+// raises the dynamic-LDS limit of the four batched kernels (call before
+// launching or capturing them)
+int batch_set_lds(const size_t lds_bytes[4]);
+int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *items,
+                       int nitems, size_t lds_bytes, hipStream_t s);
 
 // ---- refinement helpers ----------------------------------------------------
 struct OperatorDesc {
@@ -138,6 +177,8 @@ int launch_accumulate(const float *r, double *p_halo, int nx, int ny, int nz,
                       hipStream_t s);
 int launch_absmax(const double *f_halo, int nx, int ny, int nz, double *partial,
                   int *nparts, hipStream_t s);
+// partial count of launch_residual and launch_absmax
+size_t residual_blocks(int nx, int ny, int nz);
 // out[0] = max of n partial maxima (single workgroup)
 int launch_reduce_max(const double *part, int n, double *out, hipStream_t s);
 

@@ -122,8 +122,10 @@ struct flutas_psolve {
   double *P2 = nullptr;    // periodic-z second solution
   double *F = nullptr;     // copy of the right-hand side (refined mode)
   float *R32 = nullptr;    // residual (refined mode)
-  double *part = nullptr;  // reduction partials
-  size_t npart = 0;
+  // reduction partials: [0, nres) residual / absmax workgroups,
+  // [nres, nres + nx1) max|f| of the FP32 P1, then two result slots
+  double *part = nullptr;
+  size_t nres = 0, nx1 = 0;
   bool fused = false;      // four-pass pipeline (fused.hip) in use
   double *CK = nullptr;    // z elimination checkpoints of the fused pipeline
   int last_iters = 0;
@@ -239,20 +241,22 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
 // out += S(in) when accumulate.
 template <typename T, typename Tin>
 int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
-             int accumulate, T *W, hipStream_t s) {
+             int accumulate, T *W, hipStream_t s, double *absmax_part = nullptr) {
   const Geo q = halo_geo(h);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   const long long is1 = in_halo ? q.hx : nx;
   const long long is2 = in_halo ? q.hxy : (long long)nx * ny;
   if (h->fused) {
-    const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N');
-    FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ny, nz, s)));
+    // the singular column takes p(last) = 0 for NN and for periodic z
+    const int per = (h->cbc[4] == 'P');
+    const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
+    FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ny, nz, absmax_part, s)));
     FL_TRY(mark(h, "P1_x_fwd", s));
     FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
-                            h->d_a, h->d_b, h->d_c, pin, s));
+                            h->d_a, h->d_b, h->d_c, pin, per, s));
     FL_TRY(mark(h, "P2_yz_fwd", s));
     FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
-                            h->d_a, h->d_b, h->d_c, s));
+                            h->d_a, h->d_b, h->d_c, per, s));
     FL_TRY(mark(h, "P3_zy_bwd", s));
     FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, out, q.hx, q.hxy, ny, nz,
                                     (T)h->normfft, accumulate, s)));
@@ -301,12 +305,19 @@ OperatorDesc op_desc(const flutas_psolve *h) {
   return op;
 }
 
-// Device-side final reduction of the partial maxima; one 8-byte copy back.
-int reduce_max(flutas_psolve *h, int nparts, hipStream_t s, double *out) {
-  double *res = h->part + h->npart;
+// Device-side final reduction of the partial maxima (res[0] from the
+// residual partials, res[1] from the P1 partials when n1 > 0); one copy
+// back and one synchronisation.
+int reduce_max(flutas_psolve *h, int nparts, hipStream_t s, double *out,
+               int n1 = 0, double *out1 = nullptr) {
+  double *res = h->part + h->nres + h->nx1;
   FL_TRY(launch_reduce_max(h->part, nparts, res, s));
-  FL_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  if (n1 > 0) FL_TRY(launch_reduce_max(h->part + h->nres, n1, res + 1, s));
+  double host[2] = {0.0, 0.0};
+  FL_HIP(hipMemcpyAsync(host, res, (n1 > 0 ? 2 : 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   FL_HIP(hipStreamSynchronize(s));
+  *out = host[0];
+  if (n1 > 0) *out1 = host[1];
   return FLUTAS_OK;
 }
 
@@ -323,13 +334,21 @@ int solve_refined(flutas_psolve *h, const double *fsrc, double *p, hipStream_t s
   }
   const double *fin = F + q.interior;
   int np = 0;
-  FL_TRY(launch_absmax(F, h->nx, h->ny, h->nz, h->part, &np, s));
   double fmax = 0.0;
-  FL_TRY(reduce_max(h, np, s, &fmax));
-  if (!(fmax == fmax) || std::isinf(fmax)) return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+  // max|f| comes with the first FP32 transform on the fused path (P1 reads
+  // f anyway), from its own pass otherwise
+  int n1 = 0;
+  if (h->fused) {
+    n1 = (int)h->nx1;
+  } else {
+    FL_TRY(launch_absmax(F, h->nx, h->ny, h->nz, h->part, &np, s));
+    FL_TRY(reduce_max(h, np, s, &fmax));
+    if (!(fmax == fmax) || std::isinf(fmax)) return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+  }
   // initial FP32 solve
   (void)normf;
-  FL_TRY((spectral<float, double>(h, fin, true, pin, 0, h->W32, s)));
+  FL_TRY((spectral<float, double>(h, fin, true, pin, 0, h->W32, s,
+                                  n1 > 0 ? h->part + h->nres : nullptr)));
   const OperatorDesc op = op_desc(h);
   int it = 0;
   double rel = 0.0;
@@ -337,7 +356,11 @@ int solve_refined(flutas_psolve *h, const double *fsrc, double *p, hipStream_t s
     FL_TRY(launch_residual<float>(op, F, p, h->R32, h->part, &np, s));
     FL_TRY(mark(h, "residual", s));
     double rmax = 0.0;
-    FL_TRY(reduce_max(h, np, s, &rmax));
+    FL_TRY(reduce_max(h, np, s, &rmax, n1, &fmax));
+    if (n1 > 0) {
+      n1 = 0;
+      if (!(fmax == fmax) || std::isinf(fmax)) return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+    }
     rel = (fmax > 0.0) ? rmax / fmax : rmax;
     if (!(rel == rel)) return fail(FLUTAS_ENONFINITE, "residual is NaN");
     if (rel <= h->tol) break;
@@ -483,22 +506,23 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
       if (e == hipSuccess) e = hipMalloc((void **)&h->R32, N * sizeof(float));
       if (e == hipSuccess) e = hipMalloc((void **)&h->F, nhalo * sizeof(double));
     }
-    // the fused pipeline covers non-periodic z and the compiled line lengths
+    // the fused pipeline covers the compiled line lengths
     const char *env = std::getenv("FLUTAS_UNFUSED");
-    h->fused = cbc[4] != 'P' && fused_supported(h->nx) && fused_supported(h->ny) &&
+    h->fused = fused_supported(h->nx) && fused_supported(h->ny) && h->nz >= 3 &&
                !(env && env[0] == '1');
     if (h->fused) {
       const int gp = fused_group_planes(h->ny, precision == FLUTAS_FP64 ? 8 : 4);
-      // one checkpoint per plane group
+      // one checkpoint per plane group (two values per column for periodic z)
       const size_t ngrp = (h->nz + gp - 1) / gp;
-      const size_t nck = std::max<size_t>(1, ngrp);
+      const size_t nck = std::max<size_t>(1, ngrp) * (cbc[4] == 'P' ? 2 : 1);
       if (e == hipSuccess) e = hipMalloc((void **)&h->CK, (size_t)h->nx * nck * h->ny * sizeof(double));
     } else {
       if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
       if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
     }
-    h->npart = ((h->nx + 255) / 256) * (size_t)h->ny * h->nz;
-    if (e == hipSuccess) e = hipMalloc((void **)&h->part, (h->npart + 1) * sizeof(double));
+    h->nres = residual_blocks(h->nx, h->ny, h->nz);
+    h->nx1 = (h->fused && precision == FLUTAS_FP32_REFINE) ? (size_t)x_fwd_blocks(h->tx, 4, h->ny, h->nz) : 0;
+    if (e == hipSuccess) e = hipMalloc((void **)&h->part, (h->nres + h->nx1 + 2) * sizeof(double));
     if (e != hipSuccess) { rc = fail(FLUTAS_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e)); break; }
   } while (0);
   if (rc) {
@@ -712,6 +736,262 @@ int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
     G(zf, k) = G(zf, k - 1) + G(dzf, k);
     G(zc, k) = G(zf, k - 1) + G(dzc, k - 1);
   }
+  return FLUTAS_OK;
+}
+
+}  // extern "C"
+
+// ---- batched solves ---------------------------------------------------------
+// No counterpart in the reference (one solve per call there).  Each problem
+// keeps its own FP64 plan (coefficients, twiddles, work field); the batch
+// adds a device descriptor table, the item tables of the four passes, its
+// own checkpoint buffers (sized for the halved plane group) and a stream on
+// which the four launches are captured once into a hipGraph and replayed.
+struct flutas_batch {
+  int nprob = 0;
+  std::vector<flutas_psolve *> plans;
+  std::vector<BatchProblem> probs;
+  std::vector<double *> ck;
+  BatchProblem *d_probs = nullptr;
+  BatchItem *d_items[4] = {nullptr, nullptr, nullptr, nullptr};
+  int nitems[4] = {0, 0, 0, 0};
+  size_t lds[4] = {0, 0, 0, 0};
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool fields_set = false;
+};
+
+static void batch_release(flutas_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (BatchItem *d : b->d_items)
+    if (d) (void)hipFree(d);
+  if (b->d_probs) (void)hipFree(b->d_probs);
+  for (double *c : b->ck)
+    if (c) (void)hipFree(c);
+  for (flutas_psolve *h : b->plans) flutas_psolve_destroy(h);
+  b->plans.clear();
+}
+
+static int batch_upload(flutas_batch *b) {
+  FL_HIP(hipMemcpy(b->d_probs, b->probs.data(), b->probs.size() * sizeof(BatchProblem),
+                   hipMemcpyHostToDevice));
+  return FLUTAS_OK;
+}
+
+// Item tables: x passes take every (tile, plane) of every problem (tiles of
+// the problem's current tile size); slab passes take every slab, the
+// problems with the longest slabs first.  A new table drops the captured
+// graph, whose launch sizes it fixes.
+static int batch_items(flutas_batch *b) {
+  std::vector<BatchItem> items[4];
+  for (int q = 0; q < b->nprob; ++q) {
+    const BatchProblem &p = b->probs[q];
+    const int tj = batch_tile_lines(p.nx, p.tj_half);
+    const int ntile = (p.ny + tj - 1) / tj;
+    for (int k = 0; k < p.nz; ++k)
+      for (int t = 0; t < ntile; ++t) items[BATCH_X_FWD].push_back({q, t, k});
+  }
+  std::vector<int> order(b->nprob);
+  for (int q = 0; q < b->nprob; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return (long long)b->probs[x].nz * b->probs[x].ny > (long long)b->probs[y].nz * b->probs[y].ny;
+  });
+  for (int q : order)
+    for (int i = 0; i < b->probs[q].nx; ++i) items[BATCH_YZ_FWD].push_back({q, i, 0});
+  items[BATCH_ZY_BWD] = items[BATCH_YZ_FWD];
+  items[BATCH_X_BWD] = items[BATCH_X_FWD];
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (int st = 0; st < 4; ++st) {
+    if (b->d_items[st]) (void)hipFree(b->d_items[st]);
+    b->d_items[st] = nullptr;
+    b->nitems[st] = (int)items[st].size();
+    FL_HIP(hipMalloc((void **)&b->d_items[st], items[st].size() * sizeof(BatchItem)));
+    FL_HIP(hipMemcpy(b->d_items[st], items[st].data(), items[st].size() * sizeof(BatchItem),
+                     hipMemcpyHostToDevice));
+  }
+  return FLUTAS_OK;
+}
+
+static int batch_enqueue(flutas_batch *b, hipStream_t s) {
+  for (int st = 0; st < 4; ++st)
+    FL_TRY(launch_batch_stage(st, b->d_probs, b->d_items[st], b->nitems[st], b->lds[st], s));
+  return FLUTAS_OK;
+}
+
+extern "C" {
+
+int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double *dli,
+                        int nh_d, const double *const *dzci, const double *const *dzfi,
+                        flutas_batch **out) {
+  if (!out || nprob < 1 || !ng || !cbc || !dli || !dzci || !dzfi)
+    return fail(FLUTAS_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  FL_TRY(check_device());
+  flutas_batch *b = new flutas_batch;
+  b->nprob = nprob;
+  int rc = FLUTAS_OK;
+  const double zero_bc[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < nprob && rc == FLUTAS_OK; ++q) {
+    const int *n = ng + 3 * q;
+    if (!batch_supported(n[0]) || !batch_supported(n[1]) || n[2] < 3) {
+      rc = fail(FLUTAS_EUNSUPPORTED, "batched solves need nx, ny in {32,48,64,96,128,192,256}"
+                                     " and nz >= 3");
+      break;
+    }
+    flutas_psolve *h = nullptr;
+    rc = flutas_psolve_create(n, cbc + 6 * q, zero_bc, "ccc", dli + 3 * q, nh_d, dzci[q],
+                              dzfi[q], FLUTAS_FP64, 0.0, &h);
+    if (rc) break;
+    b->plans.push_back(h);
+    if (!h->fused) {
+      rc = fail(FLUTAS_EUNSUPPORTED, "problem not on the fused pipeline");
+      break;
+    }
+    BatchProblem p{};
+    p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
+    p.ncx = h->nx / 2; p.ncy = h->ny / 2;
+    p.kx_f = h->kinds[0]; p.kx_b = h->kinds[1]; p.ky_f = h->kinds[2]; p.ky_b = h->kinds[3];
+    p.per = (h->cbc[4] == 'P');
+    p.pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || p.per;
+    p.tj_half = 0; p.gp_half = 0;
+    p.scale = h->normfft;
+    p.hx = h->nx + 2;
+    p.hxy = (long long)(h->nx + 2) * (h->ny + 2);
+    p.W = h->W64;
+    p.lamx = h->d_lamx; p.lamy = h->d_lamy; p.a = h->d_a; p.b = h->d_b; p.c = h->d_c;
+    p.twx = h->tx.d_ptw; p.twhx = h->tx.d_twh; p.twqx = h->tx.d_twq;
+    p.twy = h->ty.d_ptw; p.twhy = h->ty.d_twh; p.twqy = h->ty.d_twq;
+    // checkpoints for the halved plane group (the most groups)
+    const int gp = batch_group_planes(h->ny, 1);
+    const size_t nck = (size_t)((h->nz + gp - 1) / gp) * (p.per ? 2 : 1);
+    double *ck = nullptr;
+    if (hipMalloc((void **)&ck, (size_t)h->nx * nck * h->ny * sizeof(double)) != hipSuccess) {
+      rc = fail(FLUTAS_ENOMEM, "batch checkpoint allocation");
+      break;
+    }
+    b->ck.push_back(ck);
+    p.ck = ck;
+    b->probs.push_back(p);
+    for (int st = 0; st < 4; ++st)
+      b->lds[st] = std::max(b->lds[st], batch_lds_bytes(st, (st == 0 || st == 3) ? h->nx : h->ny));
+  }
+  if (rc == FLUTAS_OK && hipMalloc((void **)&b->d_probs, nprob * sizeof(BatchProblem)) != hipSuccess)
+    rc = fail(FLUTAS_ENOMEM, "batch table allocation");
+  if (rc == FLUTAS_OK) rc = batch_items(b);
+  if (rc == FLUTAS_OK) rc = batch_set_lds(b->lds);
+  if (rc == FLUTAS_OK && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(FLUTAS_EDEVICE, "hipStreamCreateWithFlags");
+  if (rc) {
+    batch_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return FLUTAS_OK;
+}
+
+int flutas_batch_set_fields(flutas_batch *b, const double *const *f, double *const *p) {
+  if (!b || !f || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  for (int q = 0; q < b->nprob; ++q) {
+    if (!f[q] || !p[q]) return fail(FLUTAS_EINVAL, "null field pointer");
+    const long long off = 1 + b->probs[q].hx + b->probs[q].hxy;
+    b->probs[q].f = f[q] + off;
+    b->probs[q].p = p[q] + off;
+  }
+  FL_TRY(batch_upload(b));
+  b->fields_set = true;
+  return FLUTAS_OK;
+}
+
+int flutas_batch_solve(flutas_batch *b) {
+  if (!b) return fail(FLUTAS_EINVAL, "null batch");
+  if (!b->fields_set) return fail(FLUTAS_EINVAL, "flutas_batch_set_fields not called");
+  // the caller's pending work on the library stream comes first
+  FL_HIP(hipStreamSynchronize(current_stream()));
+  if (!b->exec) {
+    FL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(b, b->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(FLUTAS_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    FL_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  FL_HIP(hipGraphLaunch(b->exec, b->stream));
+  FL_HIP(hipStreamSynchronize(b->stream));
+  return FLUTAS_OK;
+}
+
+// Autotuner: per x length, the full or halved x tile; per y length, the
+// full or halved slab plane group.  Coordinate descent over the size
+// classes present, each choice timed over `reps` graph replays.
+int flutas_batch_tune(flutas_batch *b, int reps) {
+  if (!b || reps < 1) return fail(FLUTAS_EINVAL, "bad tune arguments");
+  if (!b->fields_set) return fail(FLUTAS_EINVAL, "flutas_batch_set_fields not called");
+  hipEvent_t e0, e1;
+  FL_HIP(hipEventCreate(&e0));
+  FL_HIP(hipEventCreate(&e1));
+  auto timed = [&](float *ms) -> int {
+    FL_TRY(batch_upload(b));
+    FL_TRY(batch_items(b));
+    FL_TRY(flutas_batch_solve(b));  // capture and warm
+    FL_HIP(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; ++r) FL_HIP(hipGraphLaunch(b->exec, b->stream));
+    FL_HIP(hipEventRecord(e1, b->stream));
+    FL_HIP(hipEventSynchronize(e1));
+    FL_HIP(hipEventElapsedTime(ms, e0, e1));
+    return FLUTAS_OK;
+  };
+  int rc = FLUTAS_OK;
+  for (int axis = 0; axis < 2 && rc == FLUTAS_OK; ++axis) {
+    std::vector<int> classes;
+    for (const BatchProblem &p : b->probs) classes.push_back(axis == 0 ? p.nx : p.ny);
+    std::sort(classes.begin(), classes.end());
+    classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
+    for (int n : classes) {
+      float t[2] = {0.f, 0.f};
+      for (int half = 0; half < 2 && rc == FLUTAS_OK; ++half) {
+        for (BatchProblem &p : b->probs)
+          if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_half : p.gp_half) = half;
+        rc = timed(&t[half]);
+      }
+      if (rc) break;
+      const int best = t[1] < t[0] ? 1 : 0;
+      for (BatchProblem &p : b->probs)
+        if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_half : p.gp_half) = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  FL_TRY(batch_upload(b));
+  return batch_items(b);
+}
+
+int flutas_batch_tuning(const flutas_batch *b, int *tile_half, int *group_half) {
+  if (!b) return fail(FLUTAS_EINVAL, "null batch");
+  for (int q = 0; q < b->nprob; ++q) {
+    if (tile_half) tile_half[q] = b->probs[q].tj_half;
+    if (group_half) group_half[q] = b->probs[q].gp_half;
+  }
+  return FLUTAS_OK;
+}
+
+int flutas_batch_destroy(flutas_batch *b) {
+  if (!b) return FLUTAS_OK;
+  (void)hipDeviceSynchronize();
+  batch_release(b);
+  delete b;
   return FLUTAS_OK;
 }
 

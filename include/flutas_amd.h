@@ -131,6 +131,35 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data);
 
+/* ---- batched solves (no counterpart in the reference, which solves one
+ *      problem per call; for many independent small solves) ------------- */
+typedef struct flutas_batch flutas_batch;
+/* nprob independent FP64 problems, problem q described by ng[3q..3q+2],
+ * cbc[6q..6q+5], dli[3q..3q+2] and the host metric arrays dzci[q],
+ * dzfi[q] (1-nh_d : nz+nh_d), as for flutas_psolve_create with zero BC
+ * values and c_or_f "ccc".  Every nx and ny in {32, 48, 64, 96, 128, 192,
+ * 256} and nz >= 3, else EUNSUPPORTED. */
+int flutas_batch_create(int nprob, const int *ng, const char *cbc,
+                        const double *dli, int nh_d,
+                        const double *const *dzci, const double *const *dzfi,
+                        flutas_batch **out);
+/* The halo-1 device fields of every problem: f[q] is read (interior), the
+ * interior of p[q] receives the solution.  May be called again with other
+ * fields; the captured graph is kept. */
+int flutas_batch_set_fields(flutas_batch *b, const double *const *f,
+                            double *const *p);
+/* Solves every problem.  Each of the four passes is one launch over all
+ * problems; the launches are captured into a hipGraph on the first call
+ * and replayed after.  Returns when every solution is in place. */
+int flutas_batch_solve(flutas_batch *b);
+/* Autotuner: for each x length the full or halved x tile, for each y
+ * length the full or halved slab plane group, chosen by timing `reps`
+ * replays of the whole batch per choice (the fields must be set).
+ * flutas_batch_tuning reports the choices per problem (1 = halved). */
+int flutas_batch_tune(flutas_batch *b, int reps);
+int flutas_batch_tuning(const flutas_batch *b, int *tile_half, int *group_half);
+int flutas_batch_destroy(flutas_batch *b);
+
 /* HIP events on the library stream, for host-side timing of any sequence. */
 int flutas_event_create(void **ev);
 int flutas_event_record(void *ev);

@@ -18,3 +18,9 @@ def oracle():
     from oracle import pyoracle
     pyoracle.lib()
     return pyoracle
+
+
+@pytest.fixture(scope="session")
+def api():
+    from flutas_amd import api as a
+    return a

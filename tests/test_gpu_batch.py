@@ -1,0 +1,107 @@
+"""Batched solves (flutas_batch_*): every problem of an uneven, mixed-BC
+batch must equal its own single-problem FP64 solve bit for bit (the batched
+kernels run the same device bodies), before and after autotuning and after
+new fields are bound to the captured graph; one problem is also checked
+against the oracle."""
+import numpy as np
+import pytest
+
+from cases import gauge_free, interior, relmax, z_metrics
+
+PROBLEMS = [
+    # (ng, lengths, cbc, alpha)
+    ((64, 32, 40), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((32, 96, 17), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((128, 48, 24), (1.0, 2.0, 1.0), "PPPPDD", 1.2),
+    ((256, 64, 9), (1.0, 1.0, 1.0), "PPPPPP", 0.0),
+    ((48, 192, 33), (1.0, 1.0, 2.0), "NNPPPP", 0.0),
+    ((96, 256, 3), (1.0, 1.0, 1.0), "DDNNNN", 0.0),
+    ((192, 128, 70), (1.0, 1.0, 1.0), "PPPPNN", 1.8),
+]
+
+
+def _setup(oracle, seed):
+    problems, rhs = [], []
+    rng = np.random.default_rng(seed)
+    for ng, lengths, cbc, alpha in PROBLEMS:
+        _, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], alpha, lengths[2])
+        dli = tuple(n / l for n, l in zip(ng, lengths))
+        f = np.zeros(tuple(n + 2 for n in ng), order="F")
+        f[1:-1, 1:-1, 1:-1] = rng.standard_normal(ng)
+        problems.append((ng, dli, dzci, dzfi, cbc))
+        rhs.append(f)
+    return problems, rhs
+
+
+def _single(api, problems, rhs):
+    out = []
+    for (ng, dli, dzci, dzfi, cbc), f in zip(problems, rhs):
+        plan = api.PoissonPlan(ng, dli, dzci, dzfi, cbc)
+        fd = api.DeviceArray.from_host(f)
+        pd = api.DeviceArray(fd.shape)
+        pd.zero()
+        plan.solve_into(fd, pd)
+        out.append(pd.to_host())
+        plan.destroy()
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_solves(api, oracle):
+    problems, rhs = _setup(oracle, 11)
+    ref = _single(api, problems, rhs)
+    print(f"batch of {len(problems)} problems", flush=True)
+    batch = api.BatchPlan(problems)
+    fs = [api.DeviceArray.from_host(f) for f in rhs]
+    ps = [api.DeviceArray(f.shape) for f in fs]
+    for p in ps:
+        p.zero()
+    batch.set_fields(fs, ps)
+    batch.solve()
+    for q, (p, r) in enumerate(zip(ps, ref)):
+        np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"problem {q}")
+    # a second replay of the captured graph gives the same
+    batch.solve()
+    for q, (p, r) in enumerate(zip(ps, ref)):
+        np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"replay {q}")
+    # the oracle on the first problem
+    ng, dli, dzci, dzfi, cbc = problems[0]
+    s = oracle.Solver(ng, dli, dzci, dzfi, cbc)
+    pr = rhs[0].copy(order="F")
+    s.solve(pr)
+    assert relmax(gauge_free(interior(ps[0].to_host())), gauge_free(interior(pr))) <= 1e-12
+    batch.destroy()
+
+
+@pytest.mark.gpu
+def test_batch_tuned_and_rebound(api, oracle):
+    problems, rhs = _setup(oracle, 12)
+    batch = api.BatchPlan(problems)
+    fs = [api.DeviceArray.from_host(f) for f in rhs]
+    ps = [api.DeviceArray(f.shape) for f in fs]
+    batch.set_fields(fs, ps)
+    batch.tune(reps=2)
+    th, gh = batch.tuning()
+    assert set(th) <= {0, 1} and set(gh) <= {0, 1}
+    batch.solve()
+    ref = _single(api, problems, rhs)
+    for q, (p, r) in enumerate(zip(ps, ref)):
+        np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"tuned {q}")
+    # new right-hand sides in new buffers, same captured graph
+    _, rhs2 = _setup(oracle, 13)
+    fs2 = [api.DeviceArray.from_host(f) for f in rhs2]
+    ps2 = [api.DeviceArray(f.shape) for f in fs2]
+    batch.set_fields(fs2, ps2)
+    batch.solve()
+    ref2 = _single(api, problems, rhs2)
+    for q, (p, r) in enumerate(zip(ps2, ref2)):
+        np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"rebound {q}")
+    batch.destroy()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_unsupported(api, oracle):
+    _, _, dzci, dzfi, _, _ = z_metrics(oracle, 16, 0.0, 1.0)
+    with pytest.raises(api.FlutasError) as e:
+        api.BatchPlan([((512, 64, 16), (1.0, 1.0, 1.0), dzci, dzfi, "PPPPNN")])
+    assert e.value.code == api.FLUTAS_EUNSUPPORTED

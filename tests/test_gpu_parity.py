@@ -21,12 +21,6 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "r2r_golden.npz")
 
 
-@pytest.fixture(scope="module")
-def api():
-    from flutas_amd import api as a
-    return a
-
-
 @pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01"])
 def test_fft_lines_match_golden(api, kind):
     gold = np.load(GOLD)
@@ -76,13 +70,19 @@ SOLVE_CASES = [
     ((48, 96, 30), (1.0, 1.0, 1.0), "PPPPNN", 1.5),   # radix-3 lengths
     ((40, 10, 14), (1.0, 1.0, 1.0), "DDPPDD", 0.0),   # radix-5 lengths
     ((14, 22, 12), (1.0, 1.0, 1.0), "PPNNNN", 0.0),   # generic radix 7, 11
-    # four-pass fused pipeline (compiled lengths in x and y, non-periodic z)
+    # four-pass fused pipeline (compiled lengths in x and y)
     ((64, 32, 40), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
     ((32, 64, 26), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
     ((96, 48, 20), (1.0, 1.0, 1.0), "PPPPDD", 1.2),
     ((128, 192, 18), (1.0, 2.0, 1.0), "DDNNND", 0.0),
     ((256, 64, 34), (1.0, 1.0, 1.0), "NNPPDN", 1.1),
     ((64, 64, 100), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),  # many plane groups
+    # fused periodic z (cyclic closure of gaussel_periodic in P2)
+    ((64, 32, 40), (1.0, 1.0, 2.0), "PPPPPP", 0.0),
+    ((32, 64, 26), (1.0, 1.0, 1.0), "PPPPPP", 1.3),
+    ((64, 96, 24), (1.0, 1.0, 1.0), "NNDDPP", 0.0),
+    ((128, 64, 100), (1.0, 1.0, 2.0), "PPPPPP", 0.0),  # partial last group
+    ((32, 32, 3), (1.0, 1.0, 1.0), "PPPPPP", 0.0),     # smallest cyclic system
 ]
 
 
@@ -133,11 +133,12 @@ def test_fp64_solve_matches_oracle(api, oracle, ng, lengths, cbc, alpha):
     assert plan.residual(fd, d) < 1e-12
 
 
+@pytest.mark.parametrize("cbc", ["PPPPNN", "PPPPPP"])
 @pytest.mark.parametrize("precision", [64, 32])
-def test_fused_and_line_pipelines_agree(api, oracle, monkeypatch, precision):
+def test_fused_and_line_pipelines_agree(api, oracle, monkeypatch, precision, cbc):
     """The four-pass pipeline and the five-pass line pipeline (forced with
     FLUTAS_UNFUSED=1 at plan creation) solve the same problem alike."""
-    ng, lengths, cbc, alpha = (64, 48, 40), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5
+    ng, lengths, alpha = (64, 48, 40), (2 * np.pi, np.pi, 1.0), 1.5
     _, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], alpha, lengths[2])
     dli = tuple(n / l for n, l in zip(ng, lengths))
     f = _rhs(ng, dzf, True, seed=4)

@@ -393,8 +393,8 @@ def add_interior(nx, ny, nz, pold, p):
     _check(lib.flutas_add_interior(nx, ny, nz, _dptr(pold), _dptr(p)))
 
 
-R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "RODFT01": 8,
-            "RODFT10": 9}
+R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
+            "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 
 
 def fft_lines(kind, data, n, howmany, stride=1, dist=None):

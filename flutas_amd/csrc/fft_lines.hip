@@ -5,6 +5,8 @@
 //   R2HC / HC2R          periodic          (halfcomplex order)
 //   REDFT10 / REDFT01    Neumann-Neumann   (DCT-II / DCT-III)
 //   RODFT10 / RODFT01    Dirichlet-Dirichlet (DST-II / DST-III)
+//   REDFT11 / RODFT11    Neumann-Dirichlet / Dirichlet-Neumann (DCT-IV /
+//                        DST-IV, each its own inverse up to 2n)
 // A real line of even length n is packed as N = n/2 complex values and
 // transformed with a Stockham FFT in LDS; the real split (pairs k, N-k) and
 // the cosine/sine pre/post phases (Makhoul's even/odd reordering,
@@ -213,9 +215,14 @@ __global__ void __launch_bounds__(256)
   const int l1_0 = blockIdx.x * lpb;
   const int l2 = blockIdx.y;
   const int nl = min(lpb, g.L1 - l1_0);
-  const bool fwd = (kind == K_R2HC || kind == K_REDFT10 || kind == K_RODFT10);
-  const bool trig = (kind != K_R2HC && kind != K_HC2R);
-  const bool sine = (kind == K_RODFT10 || kind == K_RODFT01);
+  // DCT-IV / DST-IV (REDFT11 / RODFT11): v_m = x_2m + i x_{n-1-2m} times
+  // exp(-i pi (4m+1) / 4n), forward FFT of length N, times exp(-i pi k/n):
+  // Y_2k = 2 Re, Y_{n-1-2k} = -2 Im.  The sine kind transforms (-1)^j x_j
+  // and reverses the output.
+  const bool quarter = (kind == K_REDFT11 || kind == K_RODFT11);
+  const bool fwd = (kind == K_R2HC || kind == K_REDFT10 || kind == K_RODFT10 || quarter);
+  const bool trig = (kind != K_R2HC && kind != K_HC2R && !quarter);
+  const bool sine = (kind == K_RODFT10 || kind == K_RODFT01 || kind == K_RODFT11);
   const bool contig = (g.ise == 1);
 
   // 1. load; forward cosine/sine kinds apply Makhoul's reordering
@@ -230,11 +237,22 @@ __global__ void __launch_bounds__(256)
     if (fwd && trig) {
       pos = (e & 1) ? (n - 1 - (e >> 1)) : (e >> 1);
       if (sine && (e & 1)) x = -x;
+    } else if (quarter) {
+      pos = (e & 1) ? (n - e) : e;  // x_2m -> Re v_m, x_{n-1-2m} -> Im v_m
+      if (sine && (e & 1)) x = -x;
     }
     sA[l * n + pos] = x;
   }
   __syncthreads();
   T *src = sA, *dst = sB;
+  if (quarter) {
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+      const int l = t / N, m = t - (t / N) * N;
+      cpx<T> *v = reinterpret_cast<cpx<T> *>(src + l * n);
+      v[m] = cmul(v[m], cpx<T>{pl.twe[2 * m], pl.twe[2 * m + 1]});
+    }
+    __syncthreads();
+  }
 
   // 2. inverse kinds: Z'_k = (X_k + conj X_{N-k}) + i e^{2 pi i k/n}(X_k - conj X_{N-k})
   if (!fwd) {
@@ -266,7 +284,20 @@ __global__ void __launch_bounds__(256)
   // 4. forward kinds: real split X_k = (Z_k + conj Z_{N-k})/2
   //    - i e^{-2 pi i k/n} (Z_k - conj Z_{N-k})/2, then halfcomplex packing
   //    (R2HC) or the phase Y_k = 2 Re(e^{-i pi k/2n} X_k) (cosine/sine).
-  if (fwd) {
+  if (quarter) {
+    for (int t = threadIdx.x; t < nl * N; t += blockDim.x) {
+      const int l = t / N, k = t - (t / N) * N;
+      const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(src + l * n);
+      T *Y = dst + l * n;
+      // exp(-i pi k / n) = twq[2k]
+      const cpx<T> u = cmul(Z[k], cpx<T>{pl.twq[4 * k], pl.twq[4 * k + 1]});
+      const int ie = 2 * k, io = n - 1 - 2 * k;
+      Y[sine ? n - 1 - ie : ie] = T(2) * u.x;
+      Y[sine ? n - 1 - io : io] = T(-2) * u.y;
+    }
+    __syncthreads();
+    T *tmp = src; src = dst; dst = tmp;
+  } else if (fwd) {
     for (int t = threadIdx.x; t < nl * (N + 1); t += blockDim.x) {
       const int l = t / (N + 1), k = t - (t / (N + 1)) * (N + 1);
       const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(src + l * n);
@@ -337,14 +368,14 @@ LinePlan<double> LineTables::dplan() const {
   LinePlan<double> p{};
   p.n = n; p.N = N; p.nst = nst;
   for (int i = 0; i < kMaxStages; ++i) p.radix[i] = radix[i];
-  p.tw = d_tw; p.twh = d_twh; p.twq = d_twq;
+  p.tw = d_tw; p.twh = d_twh; p.twq = d_twq; p.twe = d_twe;
   return p;
 }
 LinePlan<float> LineTables::fplan() const {
   LinePlan<float> p{};
   p.n = n; p.N = N; p.nst = nst;
   for (int i = 0; i < kMaxStages; ++i) p.radix[i] = radix[i];
-  p.tw = f_tw; p.twh = f_twh; p.twq = f_twq;
+  p.tw = f_tw; p.twh = f_twh; p.twq = f_twq; p.twe = f_twe;
   return p;
 }
 
@@ -356,7 +387,7 @@ int LineTables::build(int n_) {
   nst = (N == 1) ? 0 : factorize(N, radix);
   if (nst < 0) return fail(FLUTAS_EUNSUPPORTED, "n/2 has a prime factor > 64");
   const long double pi = 3.14159265358979323846264338327950288L;
-  std::vector<double> tw(2 * N), twh(2 * (N + 1)), twq(2 * (n + 1));
+  std::vector<double> tw(2 * N), twh(2 * (N + 1)), twq(2 * (n + 1)), twe(2 * N);
   for (int m = 0; m < N; ++m) {
     long double a = 2.0L * pi * m / N;
     tw[2 * m] = (double)cosl(a);
@@ -372,6 +403,11 @@ int LineTables::build(int n_) {
     twq[2 * k] = (double)cosl(a);
     twq[2 * k + 1] = (double)-sinl(a);
   }
+  for (int m = 0; m < N; ++m) {
+    long double a = pi * (4.0L * m + 1.0L) / (4.0L * n);
+    twe[2 * m] = (double)cosl(a);
+    twe[2 * m + 1] = (double)-sinl(a);
+  }
   auto up = [](const std::vector<double> &h, double **dd, float **df) -> int {
     FL_HIP(hipMalloc(dd, h.size() * sizeof(double)));
     FL_HIP(hipMemcpy(*dd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -383,15 +419,17 @@ int LineTables::build(int n_) {
   FL_TRY(up(tw, &d_tw, &f_tw));
   FL_TRY(up(twh, &d_twh, &f_twh));
   FL_TRY(up(twq, &d_twq, &f_twq));
+  FL_TRY(up(twe, &d_twe, &f_twe));
   return FLUTAS_OK;
 }
 
 void LineTables::release() {
   for (void *p : {(void *)d_tw, (void *)d_twh, (void *)d_twq, (void *)f_tw,
-                  (void *)f_twh, (void *)f_twq, (void *)d_ptw, (void *)f_ptw})
+                  (void *)f_twh, (void *)f_twq, (void *)d_ptw, (void *)f_ptw,
+                  (void *)d_twe, (void *)f_twe})
     if (p) (void)hipFree(p);
-  d_tw = d_twh = d_twq = nullptr;
-  f_tw = f_twh = f_twq = nullptr;
+  d_tw = d_twh = d_twq = d_twe = nullptr;
+  f_tw = f_twh = f_twq = f_twe = nullptr;
   d_ptw = nullptr;
   f_ptw = nullptr;
 }

@@ -38,7 +38,8 @@ int check_device();  // EDEVICE unless a gfx950 device is current
 // Transform kinds (FFTW numbering, src/fftw.f90:38-61).
 enum Kind {
   K_R2HC = 0, K_HC2R = 1,
-  K_REDFT01 = 4, K_REDFT10 = 5, K_RODFT01 = 8, K_RODFT10 = 9
+  K_REDFT01 = 4, K_REDFT10 = 5, K_REDFT11 = 6,
+  K_RODFT01 = 8, K_RODFT10 = 9, K_RODFT11 = 10
 };
 
 constexpr int kMaxStages = 16;
@@ -53,14 +54,15 @@ struct LinePlan {
   const T *tw;   // 2N values: exp(-2 pi i m / N), m < N (re, im)
   const T *twh;  // 2(N+1):  exp(-2 pi i k / n), k <= N
   const T *twq;  // 2(n+1):  exp(-i pi k / (2n)), k <= n
+  const T *twe;  // 2N:      exp(-i pi (4m+1) / (4n)), m < N (DCT-IV/DST-IV)
 };
 
 // Host-owned device tables for one length, in double and float.
 struct LineTables {
   int n = 0, N = 0, nst = 0;
   int radix[kMaxStages] = {};
-  double *d_tw = nullptr, *d_twh = nullptr, *d_twq = nullptr;
-  float *f_tw = nullptr, *f_twh = nullptr, *f_twq = nullptr;
+  double *d_tw = nullptr, *d_twh = nullptr, *d_twq = nullptr, *d_twe = nullptr;
+  float *f_tw = nullptr, *f_twh = nullptr, *f_twq = nullptr, *f_twe = nullptr;
   // per-pass twiddle tables of the fused kernels (set by
   // build_fused_twiddles when the length has a compiled plan)
   double *d_ptw = nullptr;

@@ -95,6 +95,8 @@ static int find_fft_c(char b0, char b1, int &kf, int &kb, double &norm0) {
   if (b0 == 'P' && b1 == 'P') { kf = K_R2HC; kb = K_HC2R; norm0 = 1; return 0; }
   if (b0 == 'N' && b1 == 'N') { kf = K_REDFT10; kb = K_REDFT01; norm0 = 2; return 0; }
   if (b0 == 'D' && b1 == 'D') { kf = K_RODFT10; kb = K_RODFT01; norm0 = 2; return 0; }
+  if (b0 == 'N' && b1 == 'D') { kf = K_REDFT11; kb = K_REDFT11; norm0 = 2; return 0; }
+  if (b0 == 'D' && b1 == 'N') { kf = K_RODFT11; kb = K_RODFT11; norm0 = 2; return 0; }
   return -1;
 }
 
@@ -463,7 +465,7 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
   if (find_fft_c(cbc[0], cbc[1], h->kinds[0], h->kinds[1], n0x) ||
       find_fft_c(cbc[2], cbc[3], h->kinds[2], h->kinds[3], n0y)) {
     delete h;
-    return fail(FLUTAS_EUNSUPPORTED, "ND/DN pairs in x or y (REDFT11/RODFT11) are not built");
+    return fail(FLUTAS_EINVAL, "unknown boundary-condition pair in x or y");
   }
   // normfft = 1/(norm(1)*(n+norm(2))) per direction (src/fft.f90:87,125,150)
   h->normfft = 1.0 / ((1.0 * n0x * (h->nx + 0.0)) * n0y * (h->ny + 0.0));
@@ -508,7 +510,10 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
     }
     // the fused pipeline covers the compiled line lengths
     const char *env = std::getenv("FLUTAS_UNFUSED");
-    h->fused = fused_supported(h->nx) && fused_supported(h->ny) && h->nz >= 3 &&
+    // (the tile transforms cover the R2HC, DCT-II/III and DST-II/III kinds)
+    bool quarter = false;
+    for (int kk = 0; kk < 4; ++kk) quarter |= (h->kinds[kk] == K_REDFT11 || h->kinds[kk] == K_RODFT11);
+    h->fused = fused_supported(h->nx) && fused_supported(h->ny) && h->nz >= 3 && !quarter &&
                !(env && env[0] == '1');
     if (h->fused) {
       const int gp = fused_group_planes(h->ny, precision == FLUTAS_FP64 ? 8 : 4);
@@ -639,7 +644,7 @@ int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data) {
   if (!data || howmany < 1 || stride < 1) return fail(FLUTAS_EINVAL, "bad fft_lines arguments");
   if (kind != K_R2HC && kind != K_HC2R && kind != K_REDFT01 && kind != K_REDFT10 &&
-      kind != K_RODFT01 && kind != K_RODFT10)
+      kind != K_RODFT01 && kind != K_RODFT10 && kind != K_REDFT11 && kind != K_RODFT11)
     return fail(FLUTAS_EUNSUPPORTED, "r2r kind not built");
   LineTables tab;
   FL_TRY(tab.build(n));

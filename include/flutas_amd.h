@@ -65,8 +65,9 @@ int flutas_pointer_is_device(const void *ptr, int *is_device);
  * ng: global grid (one GPU = one rank); dli = 1/(dx,dy,dz);
  * dzci, dzfi: host arrays (1-nh_d : ng[2]+nh_d).  precision: FLUTAS_FP64 or
  * FLUTAS_FP32_REFINE; tol: relative residual target of the refined mode
- * (ignored for FP64).  Unsupported here (EUNSUPPORTED): c_or_f 'f' and the
- * ND/DN pairs in x or y; odd ng (EINVAL, src/sanity.f90:155-158). */
+ * (ignored for FP64).  Unsupported here (EUNSUPPORTED): c_or_f 'f'; odd ng
+ * (EINVAL, src/sanity.f90:155-158).  ND/DN pairs in x or y use REDFT11 /
+ * RODFT11 on the five-pass line pipeline. */
 int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
                          const char c_or_f[3], const double dli[3], int nh_d,
                          const double *dzci, const double *dzfi, int precision,
@@ -127,7 +128,7 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
  * device lines of length n (element stride `stride`, line distance
  * `dist`), FP64, unnormalised FFTW definitions.  kind uses FFTW's
  * numbering (src/fftw.f90:38-61); built: R2HC(0), HC2R(1), REDFT01(4),
- * REDFT10(5), RODFT01(8), RODFT10(9); n even. */
+ * REDFT10(5), REDFT11(6), RODFT01(8), RODFT10(9), RODFT11(10); n even. */
 int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data);
 

@@ -21,7 +21,8 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "r2r_golden.npz")
 
 
-@pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01"])
+@pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01",
+                                  "REDFT11", "RODFT11"])
 def test_fft_lines_match_golden(api, kind):
     gold = np.load(GOLD)
     for n in gold["sizes"]:
@@ -83,6 +84,10 @@ SOLVE_CASES = [
     ((64, 96, 24), (1.0, 1.0, 1.0), "NNDDPP", 0.0),
     ((128, 64, 100), (1.0, 1.0, 2.0), "PPPPPP", 0.0),  # partial last group
     ((32, 32, 3), (1.0, 1.0, 1.0), "PPPPPP", 0.0),     # smallest cyclic system
+    # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
+    ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
+    ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
+    ((20, 36, 12), (1.0, 1.0, 1.0), "DNNDPP", 0.0),
 ]
 
 

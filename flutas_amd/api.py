@@ -273,8 +273,8 @@ class BatchPlan:
             for a in (f, p):
                 if a.shape != tuple(v + 2 for v in ng) or a.dtype != np.float64:
                     raise ValueError("fields must be float64 DeviceArrays of shape ng+2")
-        fv = (_VP * n)(*[f.c for f in fs])
-        pv = (_VP * n)(*[p.c for p in ps])
+        fv = (_D * n)(*[f.c for f in fs])
+        pv = (_D * n)(*[p.c for p in ps])
         _check(lib.flutas_batch_set_fields(_VP(self._h), fv, pv))
         self._fields = (list(fs), list(ps))
 

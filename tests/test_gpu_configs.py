@@ -1,0 +1,154 @@
+"""The measurement configurations at their real sizes on the GPU.
+
+  config 1  128^3 PP/PP/NN: one projection step (fillps, updt_rhs_b,
+            solve, boundp, correc) against the same step through the
+            oracle; divergence after the correction.
+  config 3  256x256x512 triply periodic two-phase step (pres_tw_src with
+            the constant-coefficient splitting, fillps, solve, correc)
+            against the oracle, plain FP64 and FP32 + refinement.
+  config 4  1024x1024x512 NN/NN/NN (4 GiB per FP64 field): too large for
+            the oracle in a test, checked through size-independent
+            properties: the relative residual of the FP64 solve, the
+            refined solve reaching 1e-10, and the agreement of the two.
+"""
+import numpy as np
+import pytest
+
+from cases import gauge_free, interior, relmax
+
+pytestmark = pytest.mark.gpu
+
+
+def _halos(case, u, v, w):
+    """Periodic halos and no-penetration z walls of a velocity field (what
+    the reference's bounduvw provides before fillps)."""
+    per = [case.cbc[2 * d] == "P" for d in range(3)]
+    for a in (u, v, w):
+        if per[0]:
+            a[0], a[-1] = a[-2], a[1]
+        if per[1]:
+            a[:, 0], a[:, -1] = a[:, -2], a[:, 1]
+        if per[2]:
+            a[:, :, 0], a[:, :, -1] = a[:, :, -2], a[:, :, 1]
+    if not per[2]:
+        w[:, :, 0] = 0.0
+        w[:, :, -2] = 0.0
+    return u, v, w
+
+
+def _velocity(case, nh_u=1, scale=1.0):
+    from flutas_amd import workloads as wl
+    return _halos(case, *wl.velocity_uniform(case, scale=scale, nh_u=nh_u))
+
+
+def test_config1_projection_step(api, oracle):
+    from flutas_amd import workloads as wl
+    case = wl.CONFIGS["cfg1_128_ppnn"]
+    nx, ny, nz = case.ng
+    dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
+    dli = case.dli
+    u, v, w = _velocity(case)
+    print(f"config 1 {case.ng} {case.cbc}", flush=True)
+    D = api.DeviceArray.from_host
+    ud, vd, wd, dzcid, dzfid = D(u), D(v), D(w), D(dzci), D(dzfi)
+    plan = api.PoissonPlan(case.ng, dli, dzci, dzfi, case.cbc)
+    pd = D(np.zeros((nx + 2, ny + 2, nz + 2), order="F"))
+    api.fillps(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzfid, 1.0, 1.0, ud, vd, wd, pd)
+    _, div0 = api.chkdiv(nx, ny, nz, dli[0], dli[1], dli[2], 1, 1, dzfid, ud, vd, wd)
+    rx, ry, rz = plan.rhsb_device()
+    api.updt_rhs_b(nx, ny, nz, "ccc", case.cbc, rx, ry, rz, pd)
+    plan.solve(pd)
+    api.boundp(case.cbc, case.ng, (0.0,) * 6, tuple(1 / d for d in dli), dzc[0], dzc[nz], pd)
+    api.correc(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzcid, 1.0, 1.0, pd, ud, vd, wd)
+    # the oracle step on the same inputs
+    po = np.zeros((nx + 2, ny + 2, nz + 2), order="F")
+    uo, vo, wo = (a.copy(order="F") for a in (u, v, w))
+    oracle.fillps(case.ng, 1, 1, dli, dzfi, 1.0, 1.0, uo, vo, wo, po)
+    s = oracle.Solver(case.ng, dli, dzci, dzfi, case.cbc)
+    s.updt_rhs_b(po)
+    s.solve(po)
+    oracle.boundp(case.cbc, case.ng, (0.0,) * 6, 1, tuple(1 / d for d in dli), dzc, dzf, po)
+    oracle.correc(case.ng, 1, 1, dli, dzci, 1.0, 1.0, po, uo, vo, wo)
+    assert relmax(interior(pd.to_host()), interior(po)) <= 1e-12
+    for g, o in ((ud, uo), (vd, vo), (wd, wo)):
+        assert relmax(interior(g.to_host()), interior(o)) <= 1e-12
+    # divergence of the corrected field (periodic halos refreshed)
+    uh, vh, wh = ud.to_host(), vd.to_host(), wd.to_host()
+    for a in (uh, vh, wh):
+        a[0], a[-1] = a[-2], a[1]
+        a[:, 0], a[:, -1] = a[:, -2], a[:, 1]
+    _, div1 = oracle.chkdiv(case.ng, dli, 1, 1, dzfi, uh, vh, wh)
+    assert div1 <= 1e-12 * div0, (div0, div1)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_config3_two_phase_step(api, oracle, precision):
+    from flutas_amd import workloads as wl
+    case = wl.CONFIGS["cfg3_256x256x512_twophase"]
+    nx, ny, nz = case.ng
+    dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
+    dli = case.dli
+    u, v, w = _velocity(case, scale=0.1)
+    g = wl.rng(case.seed + 7)
+    p = np.asfortranarray(g.standard_normal((nx + 2, ny + 2, nz + 2)))
+    pold = np.asfortranarray(g.standard_normal((nx + 2, ny + 2, nz + 2)))
+    rho = wl.sphere_density(case)
+    rho0 = float(rho.min())
+    dt = 1e-3
+    print(f"config 3 {case.ng} precision {precision}", flush=True)
+    D = api.DeviceArray.from_host
+    ud, vd, wd, pd, poldd, rhod = D(u), D(v), D(w), D(p), D(pold), D(rho)
+    dzcid, dzfid = D(dzci), D(dzfi)
+    api.pres_tw_src(nx, ny, nz, dli[0], dli[1], dli[2], 1, 1, dzcid, 1.0 / rho0, dt, dt,
+                    pd, poldd, rhod, ud, vd, wd)
+    # velocity halos after the source term, as bounduvw sets them
+    uh, vh, wh = _halos(case, ud.to_host(), vd.to_host(), wd.to_host())
+    ud, vd, wd = D(uh), D(vh), D(wh)
+    fd = D(np.zeros((nx + 2, ny + 2, nz + 2), order="F"))
+    api.fillps(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzfid, 1.0 / dt, rho0, ud, vd, wd, fd)
+    f = fd.to_host()
+    plan = api.PoissonPlan(case.ng, dli, dzci, dzfi, case.cbc, precision=precision, tol=1e-10)
+    sol = D(np.zeros_like(f))
+    plan.solve_into(fd, sol)
+    relres = plan.residual(fd, sol)
+    # oracle
+    uo, vo, wo = (a.copy(order="F") for a in (u, v, w))
+    oracle.pres_tw_src(case.ng, dli, 1, 1, dzci, 1.0 / rho0, dt, dt, p, pold, rho, uo, vo, wo)
+    _halos(case, uo, vo, wo)
+    fo = np.zeros_like(f)
+    oracle.fillps(case.ng, 1, 1, dli, dzfi, 1.0 / dt, rho0, uo, vo, wo, fo)
+    assert relmax(interior(f), interior(fo)) <= 1e-14
+    s = oracle.Solver(case.ng, dli, dzci, dzfi, case.cbc)
+    ref = fo.copy(order="F")
+    s.solve(ref)
+    got = interior(sol.to_host())
+    if precision == 64:
+        assert relres <= 1e-12, relres
+        assert relmax(got, interior(ref)) <= 1e-12
+    else:
+        iters, rel = plan.stats()
+        assert rel <= 1e-10 and relres <= 1e-10, (iters, rel, relres)
+        assert relmax(gauge_free(got), gauge_free(interior(ref))) <= 1e-8
+
+
+def test_config4_large_nn(api):
+    from flutas_amd import workloads as wl
+    case = wl.CONFIGS["cfg4_1024x1024x512_nnn"]
+    dzc, dzf, dzci, dzfi = wl.z_grid(api, case)
+    print(f"config 4 {case.ng} {case.cbc}", flush=True)
+    fd = api.DeviceArray.from_host(wl.rhs_normal(case, dzf))
+    p64 = api.DeviceArray(fd.shape)
+    p32 = api.DeviceArray(fd.shape)
+    plan = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc)
+    plan.solve_into(fd, p64)
+    assert plan.residual(fd, p64) <= 1e-12
+    plan.destroy()
+    plan32 = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc,
+                             precision=api.FP32_REFINE, tol=1e-10)
+    plan32.solve_into(fd, p32)
+    iters, rel = plan32.stats()
+    assert rel <= 1e-10 and 1 <= iters <= 4, (iters, rel)
+    plan32.destroy()
+    # both solve the same singular system with the same pinned gauge
+    a, b = p64.to_host(), p32.to_host()
+    assert relmax(gauge_free(interior(b)), gauge_free(interior(a))) <= 1e-8

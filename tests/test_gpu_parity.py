@@ -83,7 +83,7 @@ SOLVE_CASES = [
     ((32, 64, 26), (1.0, 1.0, 1.0), "PPPPPP", 1.3),
     ((64, 96, 24), (1.0, 1.0, 1.0), "NNDDPP", 0.0),
     ((128, 64, 100), (1.0, 1.0, 2.0), "PPPPPP", 0.0),  # partial last group
-    ((32, 32, 3), (1.0, 1.0, 1.0), "PPPPPP", 0.0),     # smallest cyclic system
+    ((32, 32, 4), (1.0, 1.0, 1.0), "PPPPPP", 0.0),     # smallest even cyclic system
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),

@@ -243,6 +243,56 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, Tout *__res
   }
 }
 
+// Two adjacent elements (16 bytes in FP64, 8 in FP32).
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  using type = double2;
+};
+template <>
+struct Pair<float> {
+  using type = float2;
+};
+
+// The CPT adjacent columns j = CPT*tid + u of a slab line, stored/loaded in
+// pairs when CPT is even (one wide access per pair), else one by one.
+// Loads of columns past n re-read column n-1 (branch free).
+template <typename T, int CPT, int NT>
+__device__ __forceinline__ void store_cols(T *__restrict__ line, const T *v, int n) {
+  const int j0 = CPT * threadIdx.x;
+  if constexpr (CPT % 2 == 0) {
+#pragma unroll
+    for (int u = 0; u < CPT; u += 2)
+      if (j0 + u < n) {
+        typename Pair<T>::type w;
+        w.x = v[u];
+        w.y = v[u + 1];
+        *reinterpret_cast<typename Pair<T>::type *>(line + j0 + u) = w;
+      }
+  } else {
+#pragma unroll
+    for (int u = 0; u < CPT; ++u)
+      if (j0 + u < n) line[j0 + u] = v[u];
+  }
+}
+template <typename T, int CPT, int NT>
+__device__ __forceinline__ void load_cols(const T *__restrict__ line, T *v, int n) {
+  const int j0 = CPT * threadIdx.x;
+  if constexpr (CPT % 2 == 0) {
+#pragma unroll
+    for (int u = 0; u < CPT; u += 2) {
+      const typename Pair<T>::type w =
+          *reinterpret_cast<const typename Pair<T>::type *>(line + min(j0 + u, n - 2));
+      v[u] = w.x;
+      v[u + 1] = w.y;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) v[u] = line[min(j0 + u, n - 1)];
+  }
+}
+
 // ---------------------------------------------------------------- P2 -----
 // One workgroup per x-slab i; planes are processed GP at a time.
 // Checkpoints: ck[(i*nck + g)*ny + j] = d of the last plane of group g
@@ -271,8 +321,9 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
 #pragma clang fp contract(off)
   constexpr int n = 2 * NC;  // ny
   constexpr int LS = line_stride<T, n>();
-  constexpr int CPT = (n + NT - 1) / NT;  // columns per thread
-  constexpr int LD = (GP * n + NT - 1) / NT;
+  constexpr int CPT = (n + NT - 1) / NT;  // columns per thread, adjacent
+  constexpr int LD = (GP * n / 2 + NT - 1) / NT;  // element pairs per thread
+  using V2 = typename Pair<T>::type;
   T *const A = sm, *const B = sm + GP * LS;
   // the complex FFT ends in A after an even number of passes
   constexpr bool fft_in_b = (tile::Plan<NC>::nst % 2) == 1;
@@ -290,7 +341,7 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
   double q2prev[CPT], s1[CPT], s2[CPT], gk[CPT];
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
-    const int j = threadIdx.x + u * NT;
+    const int j = CPT * threadIdx.x + u;
     lam[u] = (j < n) ? lx + lamy[j] : 0.0;
     dprev[u] = 0.0;
     pprev[u] = 0.0;
@@ -299,46 +350,16 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
     s2[u] = 0.0;
     gk[u] = 1.0;
   }
-  T r[LD];
-  // branch-free prefetch: planes past nz (last group) re-read plane nz-1
+  V2 r[LD];
+  // branch-free prefetch of element pairs (16-byte loads in FP64): planes
+  // past nz (last group) re-read plane nz-1
   auto load = [&](int g) {
     const int k0 = g * GP;
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
-      const int t = min((int)threadIdx.x + q * NT, GP * n - 1);
-      const int l = t / n, e = t - (t / n) * n;
-      r[q] = line0[(long long)min(k0 + l, nz - 1) * pstride + e];
-    }
-  };
-  // Results of a group (and its checkpoint) stay in registers and are
-  // stored in the next iteration, before the prefetch of the group after:
-  // the wait for that prefetch then also covers stores that have had a
-  // whole group of work to drain, instead of stores issued just before it.
-  T out[GP][CPT];
-  double ckout[CPT], ck2out[CPT];
-  auto store = [&](int g) {
-    const int k0 = g * GP;
-    const int np = min(GP, nz - k0);
-#pragma unroll
-    for (int l = 0; l < GP; ++l) {
-      if (l >= np) continue;
-      T *line = line0 + (long long)(k0 + l) * pstride;
-#pragma unroll
-      for (int u = 0; u < CPT; ++u) {
-        const int j = threadIdx.x + u * NT;
-        if (j < n) line[j] = out[l][u];
-      }
-    }
-    // checkpoint of group g: ck[(i*nck + g)*ny + j] (periodic z: the
-    // second right-hand side's p' follows at offset nx*nck*ny)
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-      const int j = threadIdx.x + u * NT;
-      const long long o = ((long long)i * nck + g) * n + j;
-      if (j < n) {
-        ck[o] = ckout[u];
-        if constexpr (PER) ck[o + (long long)nx * nck * n] = ck2out[u];
-      }
+      const int t = min((int)threadIdx.x + q * NT, GP * n / 2 - 1);
+      const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
+      r[q] = *reinterpret_cast<const V2 *>(line0 + (long long)min(k0 + l, nz - 1) * pstride + e);
     }
   };
   load(0);
@@ -348,18 +369,17 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
       const int t = threadIdx.x + q * NT;
-      if (t >= GP * n) continue;
-      const int l = t / n, e = t - (t / n) * n;
-      T x = r[q];
-      int pos = e;
+      if (t >= GP * n / 2) continue;
+      const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
       if (trig) {
-        pos = tile::mk_pos(e, n);
-        if (sine && (e & 1)) x = -x;
+        // Makhoul's reordering: e (even) -> e/2, e+1 -> n-1-e/2
+        A[l * LS + (e >> 1)] = r[q].x;
+        A[l * LS + n - 1 - (e >> 1)] = sine ? -r[q].y : r[q].y;
+      } else {
+        *reinterpret_cast<V2 *>(A + l * LS + e) = r[q];
       }
-      A[l * LS + pos] = x;
     }
     __syncthreads();
-    if (g > 0) store(g - 1);
     if (g + 1 < ngrp) load(g + 1);
     tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
                                             reinterpret_cast<cx<T> *>(B), W.tw);
@@ -370,9 +390,12 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
       if (l >= np) continue;
       const int k = k0 + l;
       const double ak = a[k], bk = b[k], cc = c[k];
+      // the stores go out plane by plane, overlapping the recurrence
+      T *line = line0 + (long long)k * pstride;
+      T outp[CPT];
 #pragma unroll
       for (int u = 0; u < CPT; ++u) {
-        const int j = threadIdx.x + u * NT;
+        const int j = CPT * threadIdx.x + u;
         if (j >= n) continue;
         const double y = (double)Y[l * LS + j];
         double pv;
@@ -435,17 +458,19 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
           }
         }
         pprev[u] = pv;
-        out[l][u] = (T)pv;
+        outp[u] = (T)pv;
       }
+      store_cols<T, CPT, NT>(line, outp, n);
     }
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-      ckout[u] = dprev[u];
-      ck2out[u] = q2prev[u];
+    // checkpoint of group g: ck[(i*nck + g)*ny + j] (periodic z: the
+    // second right-hand side's p' follows at offset nx*nck*ny)
+    {
+      double *cg = ck + ((long long)i * nck + g) * n;
+      store_cols<double, CPT, NT>(cg, dprev, n);
+      if constexpr (PER) store_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2prev, n);
     }
     __syncthreads();
   }
-  store(ngrp - 1);
 }
 
 // ---------------------------------------------------------------- P3 -----
@@ -464,7 +489,8 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
 #pragma clang fp contract(off)
   constexpr int n = 2 * NC;
   constexpr int LS = line_stride<T, n>();
-  constexpr int CPT = (n + NT - 1) / NT;
+  constexpr int CPT = (n + NT - 1) / NT;  // columns per thread, adjacent
+  using V2 = typename Pair<T>::type;
   T *const A = sm, *const B = sm + GP * LS;
   // pre_inv A -> B, then the FFT passes start in B
   constexpr bool fft_in_a = (tile::Plan<NC>::nst % 2) == 1;
@@ -481,7 +507,7 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
   double pnext2[CPT], plast[CPT];
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
-    const int j = threadIdx.x + u * NT;
+    const int j = CPT * threadIdx.x + u;
     lam[u] = (j < n) ? lx + lamy[j] : 0.0;
     pnext[u] = 0.0;
     pnext2[u] = 0.0;
@@ -495,41 +521,22 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
     // branch-free: out-of-range planes/columns re-read valid entries
 #pragma unroll
     for (int l = 0; l < GP; ++l)
-#pragma unroll
-      for (int u = 0; u < CPT; ++u) {
-        const int j = min((int)threadIdx.x + u * NT, n - 1);
-        pv[l][u] = line0[(long long)min(k0 + l, nz - 1) * pstride + j];
-      }
+      load_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, nz - 1) * pstride, pv[l], n);
     // the recurrence restarts at the end of group g - 1 (unused, but read
     // in bounds, for g = 0)
     const int c0 = max(g - 1, 0);
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-      const int j = min((int)threadIdx.x + u * NT, n - 1);
-      const long long o = ((long long)i * nck + c0) * n + j;
-      dck[u] = ck[o];
-      if constexpr (PER) q2ck[u] = ck[o + (long long)nx * nck * n];
-    }
+    const double *cg = ck + ((long long)i * nck + c0) * n;
+    load_cols<double, CPT, NT>(cg, dck, n);
+    if constexpr (PER) load_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2ck, n);
   };
-  constexpr int LD = (GP * n + NT - 1) / NT;
-  T outv[LD];
-  auto store = [&](int g) {
-    const int k0 = g * GP;
-    const int np = min(GP, nz - k0);
-#pragma unroll
-    for (int q = 0; q < LD; ++q) {
-      const int t = threadIdx.x + q * NT;
-      const int l = t / n, e = t - (t / n) * n;
-      if (t < GP * n && l < np) line0[(long long)(k0 + l) * pstride + e] = outv[q];
-    }
-  };
+  constexpr int LD2 = (GP * n / 2 + NT - 1) / NT;
   load(ngrp - 1);
   for (int g = ngrp - 1; g >= 0; --g) {
     const int k0 = g * GP;
     const int np = min(GP, nz - k0);
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
-      const int j = threadIdx.x + u * NT;
+      const int j = CPT * threadIdx.x + u;
       if (j >= n) continue;
       // elimination factors of this group, recomputed from the checkpoint
       double d[GP], q2[GP];
@@ -609,28 +616,27 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
       }
     }
     __syncthreads();
-    if (g < ngrp - 1) store(g + 1);
     if (g > 0) load(g - 1);
     tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
     tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
                                             reinterpret_cast<cx<T> *>(A), W.tw);
-    // the group's result goes to registers; it is stored in the next
-    // iteration, before the prefetch of the group after
+    // element pairs (16-byte stores in FP64)
 #pragma unroll
-    for (int q = 0; q < LD; ++q) {
-      const int t = min((int)threadIdx.x + q * NT, GP * n - 1);
-      const int l = t / n, e = t - (t / n) * n;
-      int pos = e;
-      T sg = T(1);
+    for (int q = 0; q < LD2; ++q) {
+      const int t = threadIdx.x + q * NT;
+      const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
+      if (t >= GP * n / 2 || l >= np) continue;
+      V2 v;
       if (trig) {
-        pos = tile::mk_pos(e, n);
-        if (sine && (e & 1)) sg = T(-1);
+        v.x = R[l * LS + (e >> 1)];
+        v.y = sine ? -R[l * LS + n - 1 - (e >> 1)] : R[l * LS + n - 1 - (e >> 1)];
+      } else {
+        v = *reinterpret_cast<const V2 *>(R + l * LS + e);
       }
-      outv[q] = R[l * LS + pos] * sg;
+      *reinterpret_cast<V2 *>(line0 + (long long)(k0 + l) * pstride + e) = v;
     }
     __syncthreads();
   }
-  store(0);
 }
 
 // P2/P3 plane groups: the two ping-pong tiles of a group stay within

@@ -408,13 +408,19 @@ __device__ __forceinline__ void pre_inv(T *sm, int kind, const T *__restrict__ t
 // cost of a second LDS tile).
 
 template <typename T, int S, int N, int SI, int L, int NT, int LSC>
-__device__ __forceinline__ void pass_pp(const cx<T> *src, cx<T> *dst,
+__device__ __forceinline__ void pass_pp(const cx<T> *__restrict__ src, cx<T> *__restrict__ dst,
                                         const T *__restrict__ tw) {
   constexpr int R = Plan<N>::r[SI];
   constexpr int NS = Plan<N>::ns(SI);
   constexpr int NB = N / R;
   constexpr int TOT = L * NB;
-  for (int t = threadIdx.x; t < TOT; t += NT) {
+  constexpr int PER = (TOT + NT - 1) / NT;
+  // fully unrolled over a thread's butterflies; src and dst never overlap,
+  // so the reads of every butterfly can be issued before the first write
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
     const int l = t / NB, b = t - (t / NB) * NB;
     const int j = b % NS;
     const cx<T> *x = src + l * LSC;
@@ -446,14 +452,20 @@ __device__ __forceinline__ void cfft_pp(cx<T> *a, cx<T> *b, const T *__restrict_
 
 // post_fwd from src (complex FFT of the packed lines) into dst.
 template <typename T, int N, int L, int NT, int LS>
-__device__ __forceinline__ void post_fwd_pp(const T *src, T *dst, int kind,
-                                            const T *__restrict__ twh,
+__device__ __forceinline__ void post_fwd_pp(const T *__restrict__ src, T *__restrict__ dst,
+                                            int kind, const T *__restrict__ twh,
                                             const T *__restrict__ twq) {
   constexpr int n = 2 * N;
   constexpr int NP = N / 2 + 1;
   constexpr int TOT = L * NP;
+  constexpr int PER = (TOT + NT - 1) / NT;
   const bool trig = is_trig(kind), sine = is_sine(kind);
-  for (int t = threadIdx.x; t < TOT; t += NT) {
+  // two items in flight per thread: their LDS reads overlap, at a bounded
+  // register cost
+#pragma unroll 2
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
     const int l = t / NP, k = t - (t / NP) * NP;
     const cx<T> *Z = reinterpret_cast<const cx<T> *>(src + l * LS);
     T *Y = dst + l * LS;
@@ -491,13 +503,19 @@ __device__ __forceinline__ void post_fwd_pp(const T *src, T *dst, int kind,
 
 // pre_inv from src (real lines) into dst (complex Z').
 template <typename T, int N, int L, int NT, int LS>
-__device__ __forceinline__ void pre_inv_pp(const T *src, T *dst, int kind,
-                                           const T *__restrict__ twh,
+__device__ __forceinline__ void pre_inv_pp(const T *__restrict__ src, T *__restrict__ dst,
+                                           int kind, const T *__restrict__ twh,
                                            const T *__restrict__ twq) {
   constexpr int NP = N / 2 + 1;
   constexpr int TOT = L * NP;
+  constexpr int PER = (TOT + NT - 1) / NT;
   const bool trig = is_trig(kind), sine = is_sine(kind);
-  for (int t = threadIdx.x; t < TOT; t += NT) {
+  // two items in flight per thread: their LDS reads overlap, at a bounded
+  // register cost
+#pragma unroll 2
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
     const int l = t / NP, k = t - (t / NP) * NP;
     const T *X = src + l * LS;
     cx<T> *Z = reinterpret_cast<cx<T> *>(dst + l * LS);

@@ -259,13 +259,58 @@ __device__ __forceinline__ int mk_pos(int e, int n) {
   return (e & 1) ? (n - 1 - (e >> 1)) : (e >> 1);
 }
 
+// Real split of the pair (k, N-k), 0 <= k <= N/2, of the complex FFT Z of
+// a packed real line (Z_N := Z_0): with A = Z_k + conj Z_{N-k},
+// B = Z_k - conj Z_{N-k} and w = exp(-2 pi i k/n),
+//   X_k = A/2 - i w B/2,   X_{N-k} = conj(A/2 + i w B/2)
+// (w_{N-k} = -conj w_k), so one twiddle product serves both members.
+template <typename T>
+__device__ __forceinline__ void split_pair(cx<T> Zk, cx<T> Zm, cx<T> w, cx<T> &Xk,
+                                           cx<T> &Xm) {
+  const cx<T> A = add(Zk, cj(Zm)), B = sub(Zk, cj(Zm));
+  const cx<T> wB = mul(w, B);
+  Xk = {T(0.5) * A.x + T(0.5) * wB.y, T(0.5) * A.y - T(0.5) * wB.x};
+  Xm = {T(0.5) * A.x - T(0.5) * wB.y, -(T(0.5) * A.y + T(0.5) * wB.x)};
+}
+// Output of spectral value X_kk (0 <= kk <= N) for the forward kinds:
+// R2HC halfcomplex (Y_kk = Re, Y_{n-kk} = Im) or the cosine/sine phase
+// (Y = 2 Re(exp(-i pi kk/2n) X), natural order, reversed for the sine kind).
+// p1 < 0 when the second value does not exist (kk = 0 or N).
+template <typename T, int N>
+__device__ __forceinline__ void fwd_out(cx<T> X, int kk, bool trig, bool sine,
+                                        const T *__restrict__ twq, T &v0, int &p0, T &v1,
+                                        int &p1) {
+  constexpr int n = 2 * N;
+  const bool two = (kk > 0 && kk < N);
+  if (!trig) {
+    v0 = X.x; p0 = kk;
+    v1 = X.y; p1 = two ? n - kk : -1;
+  } else {
+    const T c = twq[2 * kk], sn = -twq[2 * kk + 1];
+    v0 = T(2) * (X.x * c + X.y * sn);
+    v1 = T(2) * (X.x * sn - X.y * c);
+    p0 = sine ? n - 1 - kk : kk;
+    p1 = two ? (sine ? kk - 1 : n - kk) : -1;
+  }
+}
+// Inverse pre-processing of the pair (k, N-k): with A = X_k + conj X_{N-k},
+// B = X_k - conj X_{N-k} and w = exp(+2 pi i k/n),
+//   Z'_k = A + i w B,   Z'_{N-k} = conj(A - i w B).
+template <typename T>
+__device__ __forceinline__ void join_pair(cx<T> Xk, cx<T> Xm, cx<T> w, cx<T> &Zk,
+                                          cx<T> &Zm) {
+  const cx<T> A = add(Xk, cj(Xm)), B = sub(Xk, cj(Xm));
+  const cx<T> wB = mul(w, B);
+  Zk = {A.x - wB.y, A.y + wB.x};
+  Zm = {A.x + wB.y, -(A.y - wB.x)};
+}
+
 // Forward real split and kind phases, in place on L lines of n = 2N reals:
 // the complex FFT Z of the packed line becomes the R2HC halfcomplex output
 // or the REDFT10 / RODFT10 output in natural order.
 template <typename T, int N, int L, int NT, int LS>
 __device__ __forceinline__ void post_fwd(T *sm, int kind, const T *__restrict__ twh,
                                          const T *__restrict__ twq) {
-  constexpr int n = 2 * N;
   constexpr int NP = N / 2 + 1;  // pairs (k, N-k), k = 0..N/2
   constexpr int TOT = L * NP;
   constexpr int PER = (TOT + NT - 1) / NT;
@@ -279,47 +324,12 @@ __device__ __forceinline__ void post_fwd(T *sm, int kind, const T *__restrict__ 
     pos[u][0] = pos[u][1] = pos[u][2] = pos[u][3] = -1;
     if (t < TOT) {
       const int l = t / NP, k = t - (t / NP) * NP;
-      const T *line = sm + l * LS;
-      const cx<T> *Z = reinterpret_cast<const cx<T> *>(line);
-      // the two members of the pair: k and N-k (Z_N := Z_0)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int kk = (h == 0) ? k : N - k;
-        if (h == 1 && (k == 0 || 2 * k == N)) {
-          if (k == 0) {
-            // X_N = Re Z0 - Im Z0 (real), X_0 handled by h == 0
-            const cx<T> z0 = Z[0];
-            const T XN = z0.x - z0.y;
-            if (!trig) { o[u][2] = XN; pos[u][2] = N; }
-            else {
-              const T c = twq[2 * N];  // cos(pi/4)
-              const T y = T(2) * (XN * c);
-              o[u][2] = y;
-              pos[u][2] = sine ? (n - 1 - N) : N;
-            }
-          }
-          continue;
-        }
-        const cx<T> Zk = Z[kk == N ? 0 : kk];
-        const cx<T> Zm = Z[kk == 0 ? 0 : N - kk];
-        const cx<T> A = add(Zk, cj(Zm)), B = sub(Zk, cj(Zm));
-        const cx<T> wB = mul(cx<T>{twh[2 * kk], twh[2 * kk + 1]}, B);
-        const cx<T> X = {T(0.5) * A.x + T(0.5) * wB.y, T(0.5) * A.y - T(0.5) * wB.x};
-        int p0, p1;
-        T v0, v1;
-        if (!trig) {
-          v0 = X.x; p0 = kk;
-          v1 = X.y; p1 = (kk > 0 && kk < N) ? n - kk : -1;
-        } else {
-          const T c = twq[2 * kk], sn = -twq[2 * kk + 1];
-          v0 = T(2) * (X.x * c + X.y * sn);
-          v1 = T(2) * (X.x * sn - X.y * c);
-          p0 = sine ? n - 1 - kk : kk;
-          p1 = (kk > 0 && kk < N) ? (sine ? kk - 1 : n - kk) : -1;
-        }
-        o[u][2 * h] = v0; pos[u][2 * h] = p0;
-        o[u][2 * h + 1] = v1; pos[u][2 * h + 1] = p1;
-      }
+      const cx<T> *Z = reinterpret_cast<const cx<T> *>(sm + l * LS);
+      cx<T> Xk, Xm;
+      split_pair(Z[k], Z[k == 0 ? 0 : N - k], cx<T>{twh[2 * k], twh[2 * k + 1]}, Xk, Xm);
+      fwd_out<T, N>(Xk, k, trig, sine, twq, o[u][0], pos[u][0], o[u][1], pos[u][1]);
+      if (2 * k != N)
+        fwd_out<T, N>(Xm, N - k, trig, sine, twq, o[u][2], pos[u][2], o[u][3], pos[u][3]);
     }
   }
   __syncthreads();
@@ -371,22 +381,15 @@ __device__ __forceinline__ void pre_inv(T *sm, int kind, const T *__restrict__ t
     if (t < TOT) {
       const int l = t / NP, k = t - (t / NP) * NP;
       const T *X = sm + l * LS;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int kk = (h == 0) ? k : N - k;
-        if (h == 1 && (k == 0 || 2 * k == N)) continue;
-        cx<T> Xk, Xm;
-        if (!trig) {
-          Xk = hc<T, N>(X, kk);
-          Xm = hc<T, N>(X, N - kk);
-        } else {
-          Xk = vprime<T, N>(X, kk, twq, sine);
-          Xm = vprime<T, N>(X, N - kk, twq, sine);
-        }
-        const cx<T> A = add(Xk, cj(Xm)), B = sub(Xk, cj(Xm));
-        const cx<T> w = {twh[2 * kk], -twh[2 * kk + 1]};
-        z[u][h] = add(A, msi<T, 1>(mul(w, B)));
+      cx<T> Xk, Xm;
+      if (!trig) {
+        Xk = hc<T, N>(X, k);
+        Xm = hc<T, N>(X, N - k);
+      } else {
+        Xk = vprime<T, N>(X, k, twq, sine);
+        Xm = vprime<T, N>(X, N - k, twq, sine);
       }
+      join_pair(Xk, Xm, cx<T>{twh[2 * k], -twh[2 * k + 1]}, z[u][0], z[u][1]);
     }
   }
   __syncthreads();
@@ -455,7 +458,6 @@ template <typename T, int N, int L, int NT, int LS>
 __device__ __forceinline__ void post_fwd_pp(const T *__restrict__ src, T *__restrict__ dst,
                                             int kind, const T *__restrict__ twh,
                                             const T *__restrict__ twq) {
-  constexpr int n = 2 * N;
   constexpr int NP = N / 2 + 1;
   constexpr int TOT = L * NP;
   constexpr int PER = (TOT + NT - 1) / NT;
@@ -469,33 +471,17 @@ __device__ __forceinline__ void post_fwd_pp(const T *__restrict__ src, T *__rest
     const int l = t / NP, k = t - (t / NP) * NP;
     const cx<T> *Z = reinterpret_cast<const cx<T> *>(src + l * LS);
     T *Y = dst + l * LS;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int kk = (h == 0) ? k : N - k;
-      if (h == 1 && (k == 0 || 2 * k == N)) {
-        if (k == 0) {
-          const cx<T> z0 = Z[0];
-          const T XN = z0.x - z0.y;
-          if (!trig) Y[N] = XN;
-          else Y[sine ? (n - 1 - N) : N] = T(2) * (XN * twq[2 * N]);
-        }
-        continue;
-      }
-      const cx<T> Zk = Z[kk == N ? 0 : kk];
-      const cx<T> Zm = Z[kk == 0 ? 0 : N - kk];
-      const cx<T> A = add(Zk, cj(Zm)), B = sub(Zk, cj(Zm));
-      const cx<T> wB = mul(cx<T>{twh[2 * kk], twh[2 * kk + 1]}, B);
-      const cx<T> X = {T(0.5) * A.x + T(0.5) * wB.y, T(0.5) * A.y - T(0.5) * wB.x};
-      if (!trig) {
-        Y[kk] = X.x;
-        if (kk > 0 && kk < N) Y[n - kk] = X.y;
-      } else {
-        const T c = twq[2 * kk], sn = -twq[2 * kk + 1];
-        const T v0 = T(2) * (X.x * c + X.y * sn);
-        const T v1 = T(2) * (X.x * sn - X.y * c);
-        Y[sine ? n - 1 - kk : kk] = v0;
-        if (kk > 0 && kk < N) Y[sine ? kk - 1 : n - kk] = v1;
-      }
+    cx<T> Xk, Xm;
+    split_pair(Z[k], Z[k == 0 ? 0 : N - k], cx<T>{twh[2 * k], twh[2 * k + 1]}, Xk, Xm);
+    T v0, v1;
+    int p0, p1;
+    fwd_out<T, N>(Xk, k, trig, sine, twq, v0, p0, v1, p1);
+    Y[p0] = v0;
+    if (p1 >= 0) Y[p1] = v1;
+    if (2 * k != N) {
+      fwd_out<T, N>(Xm, N - k, trig, sine, twq, v0, p0, v1, p1);
+      Y[p0] = v0;
+      if (p1 >= 0) Y[p1] = v1;
     }
   }
   __syncthreads();
@@ -510,8 +496,6 @@ __device__ __forceinline__ void pre_inv_pp(const T *__restrict__ src, T *__restr
   constexpr int TOT = L * NP;
   constexpr int PER = (TOT + NT - 1) / NT;
   const bool trig = is_trig(kind), sine = is_sine(kind);
-  // two items in flight per thread: their LDS reads overlap, at a bounded
-  // register cost
 #pragma unroll 2
   for (int u = 0; u < PER; ++u) {
     const int t = threadIdx.x + u * NT;
@@ -519,22 +503,17 @@ __device__ __forceinline__ void pre_inv_pp(const T *__restrict__ src, T *__restr
     const int l = t / NP, k = t - (t / NP) * NP;
     const T *X = src + l * LS;
     cx<T> *Z = reinterpret_cast<cx<T> *>(dst + l * LS);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int kk = (h == 0) ? k : N - k;
-      if (h == 1 && (k == 0 || 2 * k == N)) continue;
-      cx<T> Xk, Xm;
-      if (!trig) {
-        Xk = hc<T, N>(X, kk);
-        Xm = hc<T, N>(X, N - kk);
-      } else {
-        Xk = vprime<T, N>(X, kk, twq, sine);
-        Xm = vprime<T, N>(X, N - kk, twq, sine);
-      }
-      const cx<T> A = add(Xk, cj(Xm)), B = sub(Xk, cj(Xm));
-      const cx<T> w = {twh[2 * kk], -twh[2 * kk + 1]};
-      Z[kk] = add(A, msi<T, 1>(mul(w, B)));
+    cx<T> Xk, Xm, Zk, Zm;
+    if (!trig) {
+      Xk = hc<T, N>(X, k);
+      Xm = hc<T, N>(X, N - k);
+    } else {
+      Xk = vprime<T, N>(X, k, twq, sine);
+      Xm = vprime<T, N>(X, N - k, twq, sine);
     }
+    join_pair(Xk, Xm, cx<T>{twh[2 * k], -twh[2 * k + 1]}, Zk, Zm);
+    Z[k] = Zk;
+    if (k != 0 && 2 * k != N) Z[N - k] = Zm;
   }
   __syncthreads();
 }

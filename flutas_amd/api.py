@@ -393,6 +393,33 @@ def add_interior(nx, ny, nz, pold, p):
     _check(lib.flutas_add_interior(nx, ny, nz, _dptr(pold), _dptr(p)))
 
 
+def load(io, filename, n, fld, halo=1):
+    """load (src/load.f90:21): 'r' reads / 'w' writes the raw global field
+    file from / into the interior of a DeviceArray with halo `halo`."""
+    ngc = (C.c_int * 3)(*[int(v) for v in n])
+    _check(lib.flutas_load(C.c_char(io.encode()), C.c_char_p(str(filename).encode()), ngc,
+                           C.c_int(halo), _dptr(fld)))
+
+
+def load_ns(io, filename, n, u, v, w, p, time=0.0, istep=0.0, halo_u=1):
+    """load_ns (src/load.f90:132): u, v, w, p and (time, istep); returns
+    (time, istep) (read back on 'r')."""
+    ngc = (C.c_int * 3)(*[int(x) for x in n])
+    t, it = C.c_double(time), C.c_double(istep)
+    _check(lib.flutas_load_ns(C.c_char(io.encode()), C.c_char_p(str(filename).encode()), ngc,
+                              C.c_int(halo_u), _dptr(u), _dptr(v), _dptr(w), _dptr(p),
+                              C.byref(t), C.byref(it)))
+    return t.value, it.value
+
+
+def load_scalar(io, filename, time=0.0, istep=0, dto=0.0):
+    """load_scalar (src/load.f90:92): returns (time, istep, dto)."""
+    t, it, d = C.c_double(time), C.c_int(istep), C.c_double(dto)
+    _check(lib.flutas_load_scalar(C.c_char(io.encode()), C.c_char_p(str(filename).encode()),
+                                  C.byref(t), C.byref(it), C.byref(d)))
+    return t.value, it.value, d.value
+
+
 R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
             "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 

@@ -161,6 +161,23 @@ int flutas_batch_tune(flutas_batch *b, int reps);
 int flutas_batch_tuning(const flutas_batch *b, int *tile_half, int *group_half);
 int flutas_batch_destroy(flutas_batch *b);
 
+/* ---- restart files in the reference's formats ------------------------- */
+/* load (src/load.f90:21-90): io 'r' reads, 'w' writes the file `filename`
+ * holding the global n[0] x n[1] x n[2] FP64 field in Fortran order (no
+ * header, no halo; the reference's MPI-IO layout at one rank) from / into
+ * the interior of the device field fld with halo `halo`.  A file of the
+ * wrong size is rejected (EINVAL), as the reference aborts. */
+int flutas_load(char io, const char *filename, const int n[3], int halo, double *fld);
+/* load_ns (src/load.f90:132-204), the CaNS layout: u, v, w (halo
+ * halo_u), p (halo 1), then the two doubles time, istep. */
+int flutas_load_ns(char io, const char *filename, const int n[3], int halo_u,
+                   double *u, double *v, double *w, double *p, double *time,
+                   double *istep);
+/* load_scalar (src/load.f90:92-130): the text file "time dto istep",
+ * written with the format (2E15.7, 1I9.8).  Host values only. */
+int flutas_load_scalar(char io, const char *filename, double *time, int *istep,
+                       double *dto);
+
 /* HIP events on the library stream, for host-side timing of any sequence. */
 int flutas_event_create(void **ev);
 int flutas_event_record(void *ev);

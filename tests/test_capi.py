@@ -84,3 +84,22 @@ def test_fortran_dropin_fails_loudly_without_gpu(tmp_path):
     if r.returncode == 0:
         pytest.skip("a gfx950 device is present")
     assert "flutas_amd: initsolver failed" in r.stdout + r.stderr
+
+
+def test_load_scalar_reference_format(tmp_path):
+    """load_scalar (src/load.f90:92-130): written with (2E15.7, 1I9.8),
+    read back list-directed."""
+    from flutas_amd import api
+    f = tmp_path / "scalar.out"
+    api.load_scalar("w", f, 1.0, 10, 1e-3)
+    assert f.read_text() == "  0.1000000E+01  0.1000000E-02 00000010\n"
+    assert api.load_scalar("r", f) == (1.0, 10, 1e-3)
+    api.load_scalar("w", f, -123.456789, 12345, 0.0)
+    assert f.read_text() == " -0.1234568E+03  0.0000000E+00 00012345\n"
+    t, it, dto = api.load_scalar("r", f)
+    assert (t, it, dto) == (-123.4568, 12345, 0.0)
+    # a list-directed line with commas, as a hand-edited restart might hold
+    f.write_text("2.5, 1.0E-4, 7\n")
+    assert api.load_scalar("r", f) == (2.5, 7, 1e-4)
+    with pytest.raises(api.FlutasError):
+        api.load_scalar("r", tmp_path / "missing.out")

@@ -649,12 +649,17 @@ constexpr int group_planes() {
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
 
-// Threads of a P2/P3 slab workgroup (y-lines of 512 points or more use
-// kSlabThreadsLong).
-constexpr int kSlabThreadsLong = 256;
-template <int NC>
+// Threads of a P2/P3 slab workgroup, as measured on 512-point y-lines: 512
+// threads (one column each, at most 128 registers, 4 waves per SIMD) for
+// the backward slab pass and for the FP32 passes, 256 for the FP64 forward
+// pass (whose cyclic and elimination state does not fit 128 registers
+// without spilling) and for every periodic-z FP32 pass; 256 below 512
+// points.
+template <typename T, int NC, bool PER, bool BWD>
 constexpr int slab_threads() {
-  return 2 * NC >= 512 ? kSlabThreadsLong : 256;
+  if (2 * NC != 512) return 256;
+  if (sizeof(T) == 8) return BWD && !PER ? 512 : 256;
+  return PER ? 256 : 512;
 }
 
 }  // namespace flutas

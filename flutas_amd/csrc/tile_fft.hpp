@@ -152,7 +152,7 @@ FL_PLAN(64, 16, 4, 1, 2)
 FL_PLAN(96, 8, 4, 3, 3)
 FL_PLAN(128, 16, 8, 1, 2)
 FL_PLAN(192, 16, 4, 3, 3)
-FL_PLAN(256, 16, 16, 1, 2)
+FL_PLAN(256, 8, 8, 4, 3)
 FL_PLAN(384, 16, 8, 3, 3)
 FL_PLAN(512, 8, 8, 8, 3)
 FL_PLAN(1024, 16, 16, 4, 3)

@@ -84,6 +84,10 @@ SOLVE_CASES = [
     ((64, 96, 24), (1.0, 1.0, 1.0), "NNDDPP", 0.0),
     ((128, 64, 100), (1.0, 1.0, 2.0), "PPPPPP", 0.0),  # partial last group
     ((32, 32, 4), (1.0, 1.0, 1.0), "PPPPPP", 0.0),     # smallest even cyclic system
+    # 512-point lines (the radix-8 plan; 512-thread backward slab pass)
+    ((512, 64, 16), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((48, 512, 12), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((64, 512, 10), (1.0, 1.0, 1.0), "DDDDPP", 0.0),
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
@@ -294,7 +298,7 @@ def test_projection_step_config1_shape(api, oracle, alpha):
 @pytest.mark.parametrize("cbc,alpha", [("PPPPNN", 1.5), ("NNNNNN", 0.0), ("PPPPPP", 0.0),
                                        ("PPPPDD", 1.3)])
 def test_fp32_refined_reaches_tolerance(api, oracle, cbc, alpha):
-    ng = (64, 48, 40)
+    ng = (64, 512, 40) if cbc == "PPPPDD" else (64, 48, 40)
     lengths = (2 * np.pi, np.pi, 1.0)
     dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], alpha, lengths[2])
     dli = tuple(n / l for n, l in zip(ng, lengths))

@@ -362,41 +362,63 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
       r[q] = *reinterpret_cast<const V2 *>(line0 + (long long)min(k0 + l, nz - 1) * pstride + e);
     }
   };
+  // The results of a group are held in registers and stored after the
+  // next group's prefetch has arrived.  Vector-memory loads and stores
+  // share one completion counter, so a wait for the prefetch also waits
+  // for every store issued before it; stores issued at the end of a
+  // group would put their full write latency on the next group's start.
+  // Branch-free: in a last group of np < GP planes, hold[l] for l >= np
+  // repeats plane np-1 and goes to the same address again.
+  T hold[GP][CPT];
+  auto flush = [&](int g) {
+    const int k0 = g * GP;
+#pragma unroll
+    for (int l = 0; l < GP; ++l)
+      store_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, nz - 1) * pstride, hold[l], n);
+    // checkpoint of group g: ck[(i*nck + g)*ny + j] (periodic z: the
+    // second right-hand side's p' follows at offset nx*nck*ny)
+    double *cg = ck + ((long long)i * nck + g) * n;
+    store_cols<double, CPT, NT>(cg, dprev, n);
+    if constexpr (PER) store_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2prev, n);
+  };
   load(0);
   for (int g = 0; g < ngrp; ++g) {
     const int k0 = g * GP;
     const int np = min(GP, nz - k0);
+    // branch-free staging: Makhoul's reordering for the cosine/sine kinds
+    // (e even -> e/2, e+1 -> n-1-e/2, odd entries negated for the sine
+    // kinds), natural order otherwise
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
       const int t = threadIdx.x + q * NT;
-      if (t >= GP * n / 2) continue;
+      if ((GP * n / 2) % NT != 0 && t >= GP * n / 2) continue;
       const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
-      if (trig) {
-        // Makhoul's reordering: e (even) -> e/2, e+1 -> n-1-e/2
-        A[l * LS + (e >> 1)] = r[q].x;
-        A[l * LS + n - 1 - (e >> 1)] = sine ? -r[q].y : r[q].y;
-      } else {
-        *reinterpret_cast<V2 *>(A + l * LS + e) = r[q];
-      }
+      const int p0 = trig ? (e >> 1) : e, p1 = trig ? n - 1 - (e >> 1) : e + 1;
+      A[l * LS + p0] = r[q].x;
+      A[l * LS + p1] = (trig && sine) ? -r[q].y : r[q].y;
     }
     __syncthreads();
-    if (g + 1 < ngrp) load(g + 1);
+    if (g > 0) flush(g - 1);
+    // the last group re-reads itself (the values are not used), so the
+    // loop carries no branch around the prefetch
+    load(min(g + 1, ngrp - 1));
     tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
                                             reinterpret_cast<cx<T> *>(B), W.tw);
     tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
     // forward elimination, plane by plane (dgtsv_homebrewed order)
 #pragma unroll
     for (int l = 0; l < GP; ++l) {
-      if (l >= np) continue;
+      if (l >= np) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) hold[l][u] = hold[l - 1][u];
+        continue;
+      }
       const int k = k0 + l;
       const double ak = a[k], bk = b[k], cc = c[k];
-      // the stores go out plane by plane, overlapping the recurrence
-      T *line = line0 + (long long)k * pstride;
-      T outp[CPT];
 #pragma unroll
       for (int u = 0; u < CPT; ++u) {
         const int j = CPT * threadIdx.x + u;
-        if (j >= n) continue;
+        if (CPT * NT != n && j >= n) continue;
         const double y = (double)Y[l * LS + j];
         double pv;
         if constexpr (!PER) {
@@ -458,19 +480,12 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
           }
         }
         pprev[u] = pv;
-        outp[u] = (T)pv;
+        hold[l][u] = (T)pv;
       }
-      store_cols<T, CPT, NT>(line, outp, n);
-    }
-    // checkpoint of group g: ck[(i*nck + g)*ny + j] (periodic z: the
-    // second right-hand side's p' follows at offset nx*nck*ny)
-    {
-      double *cg = ck + ((long long)i * nck + g) * n;
-      store_cols<double, CPT, NT>(cg, dprev, n);
-      if constexpr (PER) store_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2prev, n);
     }
     __syncthreads();
   }
+  flush(ngrp - 1);
 }
 
 // ---------------------------------------------------------------- P3 -----
@@ -529,7 +544,30 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
     load_cols<double, CPT, NT>(cg, dck, n);
     if constexpr (PER) load_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2ck, n);
   };
+  // Stores of group g's lines (element pairs, 16-byte stores in FP64) from
+  // R.  They are issued in the next group's step, after its loads have
+  // arrived: vector-memory loads and stores share one completion counter,
+  // so stores issued just before a wait for loads would add their write
+  // latency to it.
   constexpr int LD2 = (GP * n / 2 + NT - 1) / NT;
+  auto flush = [&](int g) {
+    const int k0 = g * GP;
+    const int np = min(GP, nz - k0);
+#pragma unroll
+    for (int q = 0; q < LD2; ++q) {
+      const int t = threadIdx.x + q * NT;
+      const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
+      if (((GP * n / 2) % NT != 0 && t >= GP * n / 2) || l >= np) continue;
+      V2 v;
+      if (trig) {
+        v.x = R[l * LS + (e >> 1)];
+        v.y = sine ? -R[l * LS + n - 1 - (e >> 1)] : R[l * LS + n - 1 - (e >> 1)];
+      } else {
+        v = *reinterpret_cast<const V2 *>(R + l * LS + e);
+      }
+      *reinterpret_cast<V2 *>(line0 + (long long)(k0 + l) * pstride + e) = v;
+    }
+  };
   load(ngrp - 1);
   for (int g = ngrp - 1; g >= 0; --g) {
     const int k0 = g * GP;
@@ -537,7 +575,7 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
 #pragma unroll
     for (int u = 0; u < CPT; ++u) {
       const int j = CPT * threadIdx.x + u;
-      if (j >= n) continue;
+      if (CPT * NT != n && j >= n) continue;
       // elimination factors of this group, recomputed from the checkpoint
       double d[GP], q2[GP];
       double dp = dck[u];
@@ -558,7 +596,7 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
           const int k = k0 + l;
           const double p = (k == nz - 1) ? (double)pv[l][u] : (double)pv[l][u] - d[l] * pn;
           pn = p;
-          A[l * LS + j] = (T)p;
+          pv[l][u] = (T)p;
         }
         pnext[u] = pn;
       } else {
@@ -609,34 +647,33 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
             pn2 = p2;
             p = p1 + p2 * plast[u];
           }
-          A[l * LS + j] = (T)p;
+          pv[l][u] = (T)p;
         }
         pnext[u] = pn;
         pnext2[u] = pn2;
       }
+    }
+    // the previous group's lines leave R before this group's values enter
+    // A (the same buffer)
+    if (g + 1 < ngrp) {
+      flush(g + 1);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+      const int j = CPT * threadIdx.x + u;
+      if (CPT * NT != n && j >= n) continue;
+#pragma unroll
+      for (int l = 0; l < GP; ++l)
+        A[l * LS + j] = pv[l][u];  // lines past np are not stored
     }
     __syncthreads();
     if (g > 0) load(g - 1);
     tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
     tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
                                             reinterpret_cast<cx<T> *>(A), W.tw);
-    // element pairs (16-byte stores in FP64)
-#pragma unroll
-    for (int q = 0; q < LD2; ++q) {
-      const int t = threadIdx.x + q * NT;
-      const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
-      if (t >= GP * n / 2 || l >= np) continue;
-      V2 v;
-      if (trig) {
-        v.x = R[l * LS + (e >> 1)];
-        v.y = sine ? -R[l * LS + n - 1 - (e >> 1)] : R[l * LS + n - 1 - (e >> 1)];
-      } else {
-        v = *reinterpret_cast<const V2 *>(R + l * LS + e);
-      }
-      *reinterpret_cast<V2 *>(line0 + (long long)(k0 + l) * pstride + e) = v;
-    }
-    __syncthreads();
   }
+  flush(0);
 }
 
 // P2/P3 plane groups: the two ping-pong tiles of a group stay within

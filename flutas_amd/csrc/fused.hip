@@ -208,8 +208,8 @@ int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, double *ck, int nx,
   const LinePlan<T> pl = plan_of<T>(ty);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
-    constexpr int NTP = slab_threads<T, NC, true, false>();
-    constexpr int NTN = slab_threads<T, NC, false, false>();
+    constexpr int NTP = slab_threads<T, NC, true>();
+    constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ck,
                          nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
@@ -229,8 +229,8 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
   const LinePlan<T> pl = plan_of<T>(ty);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
-    constexpr int NTP = slab_threads<T, NC, true, true>();
-    constexpr int NTN = slab_threads<T, NC, false, true>();
+    constexpr int NTP = slab_threads<T, NC, true>();
+    constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ck,
                          nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);

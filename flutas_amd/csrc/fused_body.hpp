@@ -688,14 +688,12 @@ constexpr int group_planes() {
 
 // Threads of a P2/P3 slab workgroup, as measured on 512-point y-lines: 512
 // threads (one column each, at most 128 registers, 4 waves per SIMD) for
-// the backward slab pass and for the FP32 passes, 256 for the FP64 forward
-// pass (whose cyclic and elimination state does not fit 128 registers
-// without spilling) and for every periodic-z FP32 pass; 256 below 512
+// the non-periodic passes, 256 for the periodic-z passes (whose cyclic
+// state does not fit 128 registers without spilling); 256 below 512
 // points.
-template <typename T, int NC, bool PER, bool BWD>
+template <typename T, int NC, bool PER>
 constexpr int slab_threads() {
   if (2 * NC != 512) return 256;
-  if (sizeof(T) == 8) return BWD && !PER ? 512 : 256;
   return PER ? 256 : 512;
 }
 

@@ -91,6 +91,64 @@ def cpu_baseline(case, reps=5):
                       f"{nx}x{ny}x{nz} {case.cbc} stretch-{case.alpha} solve; median of {reps}"}
 
 
+# Algorithmic bytes per grid point of the field passes around the solve in
+# one projection step (main__single_phase.f90:692-747, constant density):
+# reference order pold = p (16), fillps (32), correc (56), p = pold + p (24);
+# fused order fillps_rhs (32), correc_update (72: phi, pold, u, v, w read,
+# u, v, w, pnew written).  Face-only passes (updt_rhs_b, boundp) are not
+# counted.
+STEP_SIDE_BYTES = {"reference_order": 128, "fused": 104}
+
+
+def step_bench(api, case, dzc, dzci, dzfi, f_host, steps, warmup):
+    """One projection step around the FP64 solve at the bench size, in the
+    reference's call order and in the fused order with swapped pressure
+    buffers; ms per step from HIP events."""
+    nx, ny, nz = case.ng
+    D = api.DeviceArray.from_host
+    u, v, w = D(f_host), D(f_host), D(f_host)
+    p, pold, scratch = D(f_host), api.DeviceArray(f_host.shape), api.DeviceArray(f_host.shape)
+    dzci_d, dzfi_d = D(dzci), D(dzfi)
+    dli = case.dli
+    dl = tuple(1.0 / d for d in dli)
+    plan = api.PoissonPlan(case.ng, dli, dzci, dzfi, case.cbc)
+    rx, ry, rz = plan.rhsb_device()
+    zero_bc = (0.0,) * 6
+    dt = 1e-3
+
+    def reference_order():
+        api.copy_interior(nx, ny, nz, p, pold)
+        api.fillps(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzfi_d, 1 / dt, 1.0, u, v, w, p)
+        api.updt_rhs_b(nx, ny, nz, "ccc", case.cbc, rx, ry, rz, p)
+        plan.solve_async(p)
+        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
+        api.correc(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0, p, u, v, w)
+        api.add_interior(nx, ny, nz, pold, p)
+        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
+
+    def fused():
+        api.fillps_rhs(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzfi_d, 1 / dt, 1.0,
+                       u, v, w, "ccc", case.cbc, rx, ry, rz, scratch)
+        plan.solve_async(scratch)
+        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], scratch)
+        api.correc_update(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0,
+                          scratch, p, p, u, v, w)
+        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
+
+    t_ref = time_steps(reference_order, steps, warmup, api)
+    t_fused = time_steps(fused, steps, warmup, api)
+    t_solve = time_steps(lambda: plan.solve_async(scratch), steps, warmup, api)
+    plan.destroy()
+    npts = case.npoints
+    out = {"reference_order_ms": t_ref["mean"], "fused_ms": t_fused["mean"],
+           "solve_ms": t_solve["mean"], "side_bytes_per_point": STEP_SIDE_BYTES}
+    for key, t in (("reference_order", t_ref), ("fused", t_fused)):
+        side = t["mean"] - t_solve["mean"]
+        out[key + "_side_ms"] = side
+        out[key + "_side_GBps"] = STEP_SIDE_BYTES[key] * npts / (side * 1e6)
+    return out
+
+
 def batch_bench(api, wl, reps=5):
     """The uneven batch (64 problems, nx, ny in {64..256}, nz in {64, 128,
     256}, three BC patterns, half stretched): aggregate grid points per
@@ -157,6 +215,8 @@ def main():
     ap.add_argument("--no-refine", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the uneven 64-problem batch")
+    ap.add_argument("--no-step", action="store_true",
+                    help="skip the projection step around the solve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy the byte-counter summaries calibrate on")
     args = ap.parse_args()
@@ -254,6 +314,14 @@ def main():
         "fp64_relres": relres64,
         "fp32_refine": refine,
     }
+    if not args.no_step:
+        log("bench: projection step (reference order vs fused) ...")
+        result["projection_step"] = step_bench(api, case, dzc, dzci, dzfi, f_host,
+                                               max(3, args.steps // 2), args.warmup)
+        st = result["projection_step"]
+        log(f"bench: step {st['reference_order_ms']:.3f} ms in the reference order, "
+            f"{st['fused_ms']:.3f} ms fused (solve {st['solve_ms']:.3f} ms; side passes "
+            f"{st['reference_order_side_GBps']:.0f} / {st['fused_side_GBps']:.0f} GB/s)")
     if not args.no_batch:
         log("bench: uneven batch of 64 problems ...")
         result["batch"] = batch_bench(api, wl)

@@ -196,6 +196,12 @@ class PoissonPlan:
             raise ValueError("p must be a float64 DeviceArray of shape ng+2")
         _check(lib.flutas_psolve_solve(_VP(self._h), p.c))
 
+    def solve_async(self, p):
+        """solve, enqueued on the library stream (flutas_psolve_solve_async)."""
+        if p.shape != tuple(v + 2 for v in self.ng) or p.dtype != np.float64:
+            raise ValueError("p must be a float64 DeviceArray of shape ng+2")
+        _check(lib.flutas_psolve_solve_async(_VP(self._h), p.c))
+
     def solve_into(self, f, p):
         """Out-of-place solve: RHS from f's interior, solution into p's."""
         for a in (f, p):
@@ -383,6 +389,28 @@ def updt_rhs_b(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p):
     ptrs = [(_dptr(r) if isinstance(r, DeviceArray) else r) for r in (rhsbx, rhsby, rhsbz)]
     _check(lib.flutas_updt_rhs_b(nx, ny, nz, _chars(c_or_f, 3), _chars(_bc6(cbc), 6),
                                  *ptrs, _dptr(p)))
+
+
+def fillps_rhs(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w,
+               c_or_f, cbc, rhsbx, rhsby, rhsbz, p):
+    """fillps followed by updt_rhs_b in one pass (flutas_fillps_rhs)."""
+    ptrs = [(_dptr(r) if isinstance(r, DeviceArray) else r) for r in (rhsbx, rhsby, rhsbz)]
+    _check(lib.flutas_fillps_rhs(nx, ny, nz, nh_d, nh_u, C.c_double(dxi), C.c_double(dyi),
+                                 C.c_double(dzi), _dptr(dzfi), C.c_double(dti),
+                                 C.c_double(rho0), _dptr(u), _dptr(v), _dptr(w),
+                                 _chars(c_or_f, 3), _chars(_bc6(cbc), 6), *ptrs, _dptr(p)))
+
+
+def correc_update(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, phi, pold,
+                  pnew, u, v, w, rho=None, const_coeff=True):
+    """correc with the increment phi, then pnew = pold + phi, in one pass
+    (flutas_correc_update); pnew may be pold, not phi."""
+    rp = _dptr(rho) if rho is not None else _D()
+    _check(lib.flutas_correc_update(nx, ny, nz, nh_d, nh_u, C.c_double(dxi),
+                                    C.c_double(dyi), C.c_double(dzi), _dptr(dzci),
+                                    C.c_double(dt), C.c_double(rho0), _dptr(phi),
+                                    _dptr(pold), _dptr(pnew), _dptr(u), _dptr(v),
+                                    _dptr(w), rp, 1 if const_coeff else 0))
 
 
 def copy_interior(nx, ny, nz, src, dst):

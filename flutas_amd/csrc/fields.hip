@@ -127,6 +127,76 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---- fused prologue: fillps then updt_rhs_b in one pass -----------------
+// The face terms are added in the order updt_rhs_b adds them (x side 0,
+// x side 1, y, z), so the result equals fillps followed by updt_rhs_b bit
+// for bit.  q1..q3 are the 'f' offsets of updt_rhs_b (0 here).
+__global__ void __launch_bounds__(256)
+    k_fillps_rhs(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
+                 const double *__restrict__ dzfi, double dti, double rho0,
+                 const double *__restrict__ u, const double *__restrict__ v,
+                 const double *__restrict__ w, int q1, int q2, int q3,
+                 const double *__restrict__ rx, const double *__restrict__ ry,
+                 const double *__restrict__ rz, double *__restrict__ p) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const double dtidxi = dti * dxi, dtidyi = dti * dyi;
+  const long long c = ixh(i, j, k, nh_u, nx, ny);
+  double val =
+      (((w[c] - w[ixh(i, j, k - 1, nh_u, nx, ny)]) * dti * dzfi[k + nh_d - 1] +
+        (v[c] - v[ixh(i, j - 1, k, nh_u, nx, ny)]) * dtidyi) +
+       (u[c] - u[c - 1]) * dtidxi) *
+      rho0;
+  const long long mx = (j - 1) + (long long)ny * (k - 1);
+  const long long my = (i - 1) + (long long)nx * (k - 1);
+  const long long mz = (i - 1) + (long long)nx * (j - 1);
+  if (i == 1) val += rx[mx];
+  if (i == nx - q1) val += rx[mx + (long long)ny * nz];
+  if (j == 1) val += ry[my];
+  if (j == ny - q2) val += ry[my + (long long)nx * nz];
+  if (k == 1) val += rz[mz];
+  if (k == nz - q3) val += rz[mz + (long long)nx * ny];
+  p[ixh(i, j, k, 1, nx, ny)] = val;
+}
+
+// ---- fused epilogue: correc, then pnew = pold + phi ---------------------
+// phi is the solver's increment (halos set by boundp) and is only read, so
+// the pass has no neighbour hazards; pnew may be pold (in place) but not
+// phi.  Same arithmetic as correc and the increment update.
+__global__ void __launch_bounds__(256)
+    k_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
+                    double factorj, const double *__restrict__ dzci, double dt,
+                    double rho0i, const double *__restrict__ phi,
+                    const double *pold, double *pnew, double *__restrict__ u,
+                    double *__restrict__ v, double *__restrict__ w,
+                    const double *__restrict__ rho, int cc) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y + 1, k = blockIdx.z + 1;
+  if (i > nx) return;
+  const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
+  const long long ip = c1 + 1, jp = c1 + (nx + 2),
+                  kp = c1 + (long long)(nx + 2) * (ny + 2);
+  const double pc = phi[c1];
+  const double fz = dt * dzci[k + nh_d - 1];
+  const double po = pold[c1];
+  if (cc) {
+    u[cu] = u[cu] - factori * (phi[ip] - pc) * rho0i;
+    v[cu] = v[cu] - factorj * (phi[jp] - pc) * rho0i;
+    w[cu] = w[cu] - fz * (phi[kp] - pc) * rho0i;
+  } else {
+    const double rc = rho[c1];
+    const double rhox = 0.5 * (rho[ip] + rc), rhoy = 0.5 * (rho[jp] + rc),
+                 rhoz = 0.5 * (rho[kp] + rc);
+    u[cu] = u[cu] - factori * (phi[ip] - pc) / rhox;
+    v[cu] = v[cu] - factorj * (phi[jp] - pc) / rhoy;
+    w[cu] = w[cu] - fz * (phi[kp] - pc) / rhoz;
+  }
+  pnew[c1] = po + pc;
+}
+
 // ---- chkdiv (src/chkdiv.f90:38-60): per-block partial sum and max -------
 constexpr int kRedThreads = 256;
 
@@ -582,6 +652,48 @@ int flutas_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
   hipLaunchKernelGGL(k_add_face, dim3((ny + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 0, q1, rhsbx, p);
   hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 1, q2, rhsby, p);
   hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, ny), dim3(256), 0, s, nx, ny, nz, 2, q3, rhsbz, p);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_fillps_rhs(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                      double dyi, double dzi, const double *dzfi, double dti,
+                      double rho0, const double *u, const double *v,
+                      const double *w, const char c_or_f[3], const char cbc[6],
+                      const double *rhsbx, const double *rhsby,
+                      const double *rhsbz, double *p) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzfi || !u || !v || !w || !c_or_f || !cbc || !rhsbx || !rhsby || !rhsbz || !p)
+    return fail(FLUTAS_EINVAL, "null pointer");
+  const int q1 = (c_or_f[0] == 'f' && cbc[1] == 'D') ? 1 : 0;
+  const int q2 = (c_or_f[1] == 'f' && cbc[3] == 'D') ? 1 : 0;
+  const int q3 = (c_or_f[2] == 'f' && cbc[5] == 'D') ? 1 : 0;
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_fillps_rhs, grid3(nx, ny, nz, bx), dim3(bx), 0, current_stream(), nx,
+                     ny, nz, nh_d, nh_u, dxi, dyi, dzfi, dti, rho0, u, v, w, q1, q2, q3, rhsbx,
+                     rhsby, rhsbz, p);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                         double dyi, double dzi, const double *dzci, double dt,
+                         double rho0, const double *phi, const double *pold,
+                         double *pnew, double *u, double *v, double *w,
+                         const double *rho, int const_coeff) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzci || !phi || !pold || !pnew || !u || !v || !w || (!const_coeff && !rho))
+    return fail(FLUTAS_EINVAL, "null pointer");
+  if (pnew == phi) return fail(FLUTAS_EINVAL, "pnew must not be the increment array phi");
+  const double rho0i = 1.0 / rho0, factori = dt * dxi, factorj = dt * dyi;
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_correc_update, grid3(nx, ny, nz, bx), dim3(bx), 0, current_stream(), nx,
+                     ny, nz, nh_d, nh_u, factori, factorj, dzci, dt, rho0i, phi, pold, pnew, u,
+                     v, w, rho, const_coeff ? 1 : 0);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }

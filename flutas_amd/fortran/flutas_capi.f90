@@ -134,6 +134,25 @@ module flutas_capi
       type(c_ptr), value :: dzci, p, u, v, w, rho
       integer(c_int) :: rc
     end function
+    ! fused prologue (fillps + updt_rhs_b) and epilogue (correc + pnew = pold + phi)
+    function flutas_fillps_rhs(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w, &
+        c_or_f, cbc, rhsbx, rhsby, rhsbz, p) bind(C, name="flutas_fillps_rhs") result(rc)
+      import :: c_int, c_double, c_char, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u
+      real(c_double), value :: dxi, dyi, dzi, dti, rho0
+      type(c_ptr), value :: dzfi, u, v, w
+      character(kind=c_char), dimension(*) :: c_or_f, cbc
+      type(c_ptr), value :: rhsbx, rhsby, rhsbz, p
+      integer(c_int) :: rc
+    end function
+    function flutas_correc_update(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, phi, &
+        pold, pnew, u, v, w, rho, const_coeff) bind(C, name="flutas_correc_update") result(rc)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u, const_coeff
+      real(c_double), value :: dxi, dyi, dzi, dt, rho0
+      type(c_ptr), value :: dzci, phi, pold, pnew, u, v, w, rho
+      integer(c_int) :: rc
+    end function
     function flutas_chkdiv(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w, divtot, divmax) &
         bind(C, name="flutas_chkdiv") result(rc)
       import :: c_int, c_double, c_ptr

@@ -231,6 +231,33 @@ int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
                   const double *v, const double *w, double *divtot,
                   double *divmax);
 
+/* Fused prologue of the projection step: fillps then updt_rhs_b
+ * (src/fillps.f90:16-26 with src/bound.f90:829-944, called back to back in
+ * main__single_phase.f90:707-709) in one pass; bit-identical to the two
+ * calls.  rhsb* are device arrays (flutas_psolve_rhsb_device). */
+int flutas_fillps_rhs(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                      double dyi, double dzi, const double *dzfi, double dti,
+                      double rho0, const double *u, const double *v,
+                      const double *w, const char c_or_f[3], const char cbc[6],
+                      const double *rhsbx, const double *rhsby,
+                      const double *rhsbz, double *p);
+
+/* Fused epilogue of the projection step: correc (src/correc.f90:16-29) with
+ * the increment phi, then pnew = pold + phi on the interior (the correc call
+ * and the update loop of main__single_phase.f90:726, :734-740), in one pass; bit-identical to
+ * flutas_correc followed by flutas_add_interior.  The new pressure goes to
+ * pnew, which may be pold but not phi, so phi stays readable for the
+ * neighbours of every point; a caller that swaps its pressure buffers
+ * afterwards (pnew becomes the next step's pold) also saves the pold = p
+ * copy.  phi's halos must be set (flutas_boundp), as for correc; pnew's
+ * halos are not written (flutas_boundp on pnew, as after the reference's
+ * update). */
+int flutas_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                         double dyi, double dzi, const double *dzci, double dt,
+                         double rho0, const double *phi, const double *pold,
+                         double *pnew, double *u, double *v, double *w,
+                         const double *rho, int const_coeff);
+
 /* pold = p on the interior (main__two_phase_inc_isot.f90:656-673). */
 int flutas_copy_interior(int nx, int ny, int nz, const double *src, double *dst);
 

@@ -19,12 +19,16 @@ __device__ __forceinline__ long long ixh(int i, int j, int k, int h, int nx,
              ((long long)(j - 1 + h) + (long long)(ny + 2 * h) * (k - 1 + h));
 }
 
+// Point kernels: one workgroup per (j, k) row of up to kRowThreads points
+// (one lane per point; a 512-point row is one 512-lane workgroup, which
+// streams faster than two 256-lane halves).
+constexpr int kRowThreads = 1024;
 static inline dim3 grid3(int nx, int ny, int nz, int bx) {
   return dim3((nx + bx - 1) / bx, ny, nz);
 }
 
 // ---- fillps (src/fillps.f90:42-61, _CONSTANT_COEFFS_POISSON) -------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
              const double *__restrict__ dzfi, double dti, double rho0,
              const double *__restrict__ u, const double *__restrict__ v,
@@ -43,7 +47,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- pres_tw_src (src/source.f90:272-306) -------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi, int nh_d,
                   int nh_u, const double *__restrict__ dzci, double rho0i,
                   double f_t12, double f1, double f2,
@@ -79,7 +83,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- pres_sp_src (src/source.f90:326-343) -------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi, double dyi,
                   int nh_d, int nh_u, const double *__restrict__ dzci,
                   double rho0i, const double *__restrict__ pold,
@@ -98,7 +102,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- correc (src/correc.f90:120-144) ------------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_correc(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
              double factorj, const double *__restrict__ dzci, double dt,
              double rho0i, const double *__restrict__ p, double *__restrict__ u,
@@ -131,7 +135,7 @@ __global__ void __launch_bounds__(256)
 // The face terms are added in the order updt_rhs_b adds them (x side 0,
 // x side 1, y, z), so the result equals fillps followed by updt_rhs_b bit
 // for bit.  q1..q3 are the 'f' offsets of updt_rhs_b (0 here).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_fillps_rhs(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
                  const double *__restrict__ dzfi, double dti, double rho0,
                  const double *__restrict__ u, const double *__restrict__ v,
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(256)
 // phi is the solver's increment (halos set by boundp) and is only read, so
 // the pass has no neighbour hazards; pnew may be pold (in place) but not
 // phi.  Same arithmetic as correc and the increment update.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
                     double factorj, const double *__restrict__ dzci, double dt,
                     double rho0i, const double *__restrict__ phi,
@@ -281,7 +285,7 @@ __global__ void k_add_face(int nx, int ny, int nz, int dir, int q,
 }
 
 // ---- interior copies ------------------------------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kRowThreads)
     k_interior(int nx, int ny, int nz, const double *__restrict__ src,
                double *__restrict__ dst, int add) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -470,7 +474,7 @@ int launch_reduce_max(const double *part, int n, double *out, hipStream_t s) {
   return FLUTAS_OK;
 }
 
-static int xblock(int nx) { return std::min(256, ((nx + 63) / 64) * 64); }
+static int xblock(int nx) { return std::min(kRowThreads, ((nx + 63) / 64) * 64); }
 
 template <typename Tr>
 int launch_residual(const OperatorDesc &op, const double *f, const double *p,

@@ -353,24 +353,40 @@ __global__ void __launch_bounds__(256)
     auto gather = [&](int k) {
       const long long c = c0 + sz * (k + 1);
       Plane7 v;
+      // unconditional loads at selected addresses (a non-periodic boundary
+      // neighbour re-reads the centre; ghost() replaces it)
+      const long long ixm = (i > 0) ? c - 1 : ((op.cbc[0] == 'P') ? c + (nx - 1) : c);
+      const long long ixp = (i < nx - 1) ? c + 1 : ((op.cbc[1] == 'P') ? c - (nx - 1) : c);
+      const long long iym = (j > 0) ? c - sy : ((op.cbc[2] == 'P') ? c + sy * (ny - 1) : c);
+      const long long iyp = (j < ny - 1) ? c + sy : ((op.cbc[3] == 'P') ? c - sy * (ny - 1) : c);
       v.f = f[c];
       v.c = p[c];
-      v.xm = (i > 0) ? p[c - 1] : ((op.cbc[0] == 'P') ? p[c + (nx - 1)] : 0.0);
-      v.xp = (i < nx - 1) ? p[c + 1] : ((op.cbc[1] == 'P') ? p[c - (nx - 1)] : 0.0);
-      v.ym = (j > 0) ? p[c - sy] : ((op.cbc[2] == 'P') ? p[c + sy * (ny - 1)] : 0.0);
-      v.yp = (j < ny - 1) ? p[c + sy] : ((op.cbc[3] == 'P') ? p[c - sy * (ny - 1)] : 0.0);
+      v.xm = p[ixm];
+      v.xp = p[ixp];
+      v.ym = p[iym];
+      v.yp = p[iyp];
       return v;
     };
     auto ghost = [&](double nb, double pc, char bc, bool inside) {
       if (inside || bc == 'P') return nb;
       return bc == 'N' ? pc : -pc;
     };
+    // plane index of the walk's neighbour after plane k (periodic wrap; a
+    // non-periodic last plane re-reads itself, unused)
+    auto knext = [&](int k) { return (k + 1 < nz) ? k + 1 : (zper ? k + 1 - nz : nz - 1); };
     double pzm = (k0 > 0) ? p[c0 + sz * k0] : (zper ? p[c0 + sz * nz] : 0.0);
+    // two planes in flight: plane k+1 is loaded while plane k is evaluated
+    // and plane k+2 is requested; the residuals are held in registers and
+    // stored after the walk, so no store shares the load counter inside it
     Plane7 cur = gather(k0);
-    for (int k = k0; k < k1; ++k) {
-      // plane k+1 (the z neighbour, and the next plane of the walk)
-      const int kn = (k + 1 < nz) ? k + 1 : (zper ? 0 : k);
-      const Plane7 nxt = gather(kn);
+    Plane7 nxt = gather(knext(k0));
+    Tr hold[kResidualPlanes];
+#pragma unroll
+    for (int q = 0; q < kResidualPlanes; ++q) {
+      const int k = k0 + q;
+      if (k >= k1) continue;
+      Plane7 nxt2 = nxt;
+      if (k + 1 < k1) nxt2 = gather(knext(k + 1));
       const double pc = cur.c;
       const double lx = (ghost(cur.xp, pc, op.cbc[1], i < nx - 1) - 2.0 * pc +
                          ghost(cur.xm, pc, op.cbc[0], i > 0)) * op.dx2;
@@ -381,11 +397,17 @@ __global__ void __launch_bounds__(256)
       if (k > 0 || zper) lz += op.a[k] * pzm;
       if (k < nz - 1 || zper) lz += op.c[k] * pzp;
       const double res = cur.f - ((lx + ly) + lz);
-      if (r) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
+      hold[q] = (Tr)res;
       const double ar = fabs(res);
       if (ar > am || res != res) am = (res != res) ? res : ar;
       pzm = pc;
       cur = nxt;
+      nxt = nxt2;
+    }
+    if (r) {
+#pragma unroll
+      for (int q = 0; q < kResidualPlanes; ++q)
+        if (k0 + q < k1) r[i + (long long)nx * (j + (long long)ny * (k0 + q))] = hold[q];
     }
   }
   smax[threadIdx.x] = am;
@@ -448,28 +470,40 @@ int launch_accumulate(const float *r, double *p, int nx, int ny, int nz,
 }
 
 // max over n partial maxima (NaN propagates), one workgroup
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     k_reduce_max(const double *__restrict__ part, int n, double *__restrict__ out) {
-  __shared__ double smax[256];
-  double m = 0.0;
-  for (int t = threadIdx.x; t < n; t += 256) {
-    const double o = part[t];
-    if (o > m || o != o) m = o;
+  __shared__ double wmax[16];
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  auto take = [](double &acc, double o) {
+    if (o > acc || o != o) acc = o;
+  };
+  // four independent loads in flight per step
+  int t = threadIdx.x;
+  for (; t + 3 * 1024 < n; t += 4 * 1024) {
+    const double o0 = part[t], o1 = part[t + 1024], o2 = part[t + 2048], o3 = part[t + 3072];
+    take(m[0], o0);
+    take(m[1], o1);
+    take(m[2], o2);
+    take(m[3], o3);
   }
-  smax[threadIdx.x] = m;
+  for (; t < n; t += 1024) take(m[0], part[t]);
+  take(m[0], m[1]);
+  take(m[2], m[3]);
+  take(m[0], m[2]);
+  double am = m[0];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) take(am, __shfl_xor(am, off));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = am;
   __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      const double o = smax[threadIdx.x + off];
-      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
-    }
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wmax[0];
+    for (int w = 1; w < 16; ++w) take(r, wmax[w]);
+    out[0] = r;
   }
-  if (threadIdx.x == 0) out[0] = smax[0];
 }
 
 int launch_reduce_max(const double *part, int n, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, s, part, n, out);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, part, n, out);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }

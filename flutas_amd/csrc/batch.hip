@@ -13,6 +13,9 @@
 
 namespace flutas {
 
+// 256 threads for every pass, as the single-problem kernels use at these
+// line lengths (tile_threads, slab_threads), so that a problem solved in a
+// batch runs the same instantiated bodies as its single solve.
 constexpr int kBatchThreads = 256;
 
 #define FL_BATCH_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128)

@@ -8,7 +8,7 @@ namespace flutas {
 using tile::cx;
 
 template <typename T, typename Tin, int NC, int TJ, int NT>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_x_fwd(const Tin *__restrict__ in, long long is1, long long is2,
             T *__restrict__ Wt, int ny, int kind, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq,
@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(NT)
 }
 
 template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_x_bwd(const T *__restrict__ Wt, Tout *__restrict__ out, long long os1,
             long long os2, int ny, int kind, T scale, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq) {
@@ -136,8 +136,8 @@ int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  long long is2, T *Wt, int ny, int nz, double *absmax_part,
                  hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
-  constexpr int NT = 256;
   FL_DISPATCH(tx.n / 2, {
+    constexpr int NT = tile_threads<NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
@@ -152,8 +152,8 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
                  long long os1, long long os2, int ny, int nz, T scale,
                  int accumulate, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
-  constexpr int NT = 256;
   FL_DISPATCH(tx.n / 2, {
+    constexpr int NT = tile_threads<NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     if (accumulate)

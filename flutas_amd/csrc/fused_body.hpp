@@ -116,10 +116,18 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
   }
 }
 
-// Minimum waves per SIMD of a P2/P3 workgroup size: two 512-thread slab
-// workgroups per CU (4 waves per SIMD, at most 128 VGPRs each), since the
-// grid has only one workgroup per x-slab.
+// Minimum waves per SIMD of a workgroup size: two 512-thread workgroups
+// per CU (4 waves per SIMD, at most 128 VGPRs each), the most that the LDS
+// of two tiles or slabs allows.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
+
+// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and more
+// (measured faster than 256 for the transposed loads of P4), 256 below
+// (faster for the short lines of batched problems).
+template <int NC>
+constexpr int tile_threads() {
+  return 2 * NC >= 512 ? 512 : 256;
+}
 
 // ---------------------------------------------------------------- P1 -----
 // One workgroup: TJ consecutive x-lines (j0..j0+TJ-1) at one k.

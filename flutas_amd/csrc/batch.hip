@@ -13,9 +13,11 @@
 
 namespace flutas {
 
-// 256 threads for every pass, as the single-problem kernels use at these
-// line lengths (tile_threads, slab_threads), so that a problem solved in a
-// batch runs the same instantiated bodies as its single solve.
+// Tile passes run 256 threads, as the single-problem kernels at these line
+// lengths; slab passes one thread per column (batch_slab_threads), where
+// the single-problem kernels run 256, so a batched solve may differ from
+// the single solve in the last bits (different unrolling of the same
+// arithmetic).
 constexpr int kBatchThreads = 256;
 
 #define FL_BATCH_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128)
@@ -87,60 +89,46 @@ __global__ void __launch_bounds__(kBatchThreads)
 
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_yz(const BatchProblem &q, int i, double *dyn) {
-  yz_fwd_slab<double, NC, GP, kBatchThreads, PER>(
+  yz_fwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
       q.W, q.ck, q.nx, q.nz, q.ky_f, q.lamx, q.lamy, q.a, q.b, q.c, q.pin, q.twy, q.twhy,
       q.twqy, dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_zy(const BatchProblem &q, int i, double *dyn) {
-  zy_bwd_slab<double, NC, GP, kBatchThreads, PER>(
+  zy_bwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
       q.W, q.ck, q.nx, q.nz, q.ky_b, q.lamx, q.lamy, q.a, q.b, q.c, q.twy, q.twhy, q.twqy,
       dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }
 
-__global__ void __launch_bounds__(kBatchThreads)
+// The slab passes are launched once per y-line length (every item of a
+// launch has ny = 2*NC), with batch_slab_threads<NC>() threads.
+template <int NC>
+__global__ void __launch_bounds__(batch_slab_threads<NC>())
     k_batch_yz_fwd(const BatchProblem *__restrict__ P, const BatchItem *__restrict__ items) {
   extern __shared__ __align__(16) double dyn[];
   const BatchItem w = items[blockIdx.x];
   const BatchProblem &q = P[w.prob];
-  switch (q.ncy) {
-#define FL_CASE(NCV)                                                                       \
-  case NCV:                                                                                \
-    if (q.gp_half) {                                                                       \
-      if (q.per) batch_yz<NCV, gp_variant<NCV>(1), true>(q, w.u, dyn);                     \
-      else batch_yz<NCV, gp_variant<NCV>(1), false>(q, w.u, dyn);                          \
-    } else {                                                                               \
-      if (q.per) batch_yz<NCV, gp_variant<NCV>(0), true>(q, w.u, dyn);                     \
-      else batch_yz<NCV, gp_variant<NCV>(0), false>(q, w.u, dyn);                          \
-    }                                                                                      \
-    break;
-    FL_BATCH_SIZES(FL_CASE)
-#undef FL_CASE
-    default:
-      break;
+  if (q.gp_half) {
+    if (q.per) batch_yz<NC, gp_variant<NC>(1), true>(q, w.u, dyn);
+    else batch_yz<NC, gp_variant<NC>(1), false>(q, w.u, dyn);
+  } else {
+    if (q.per) batch_yz<NC, gp_variant<NC>(0), true>(q, w.u, dyn);
+    else batch_yz<NC, gp_variant<NC>(0), false>(q, w.u, dyn);
   }
 }
 
-__global__ void __launch_bounds__(kBatchThreads)
+template <int NC>
+__global__ void __launch_bounds__(batch_slab_threads<NC>())
     k_batch_zy_bwd(const BatchProblem *__restrict__ P, const BatchItem *__restrict__ items) {
   extern __shared__ __align__(16) double dyn[];
   const BatchItem w = items[blockIdx.x];
   const BatchProblem &q = P[w.prob];
-  switch (q.ncy) {
-#define FL_CASE(NCV)                                                                       \
-  case NCV:                                                                                \
-    if (q.gp_half) {                                                                       \
-      if (q.per) batch_zy<NCV, gp_variant<NCV>(1), true>(q, w.u, dyn);                     \
-      else batch_zy<NCV, gp_variant<NCV>(1), false>(q, w.u, dyn);                          \
-    } else {                                                                               \
-      if (q.per) batch_zy<NCV, gp_variant<NCV>(0), true>(q, w.u, dyn);                     \
-      else batch_zy<NCV, gp_variant<NCV>(0), false>(q, w.u, dyn);                          \
-    }                                                                                      \
-    break;
-    FL_BATCH_SIZES(FL_CASE)
-#undef FL_CASE
-    default:
-      break;
+  if (q.gp_half) {
+    if (q.per) batch_zy<NC, gp_variant<NC>(1), true>(q, w.u, dyn);
+    else batch_zy<NC, gp_variant<NC>(1), false>(q, w.u, dyn);
+  } else {
+    if (q.per) batch_zy<NC, gp_variant<NC>(0), true>(q, w.u, dyn);
+    else batch_zy<NC, gp_variant<NC>(0), false>(q, w.u, dyn);
   }
 }
 
@@ -202,32 +190,42 @@ size_t batch_lds_bytes(int stage, int n) {
 int batch_set_lds(const size_t lds_bytes[4]) {
   FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_fwd,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[0]));
-  FL_HIP(hipFuncSetAttribute((const void *)k_batch_yz_fwd,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[1]));
-  FL_HIP(hipFuncSetAttribute((const void *)k_batch_zy_bwd,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[2]));
   FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_bwd,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[3]));
+#define FL_CASE(NCV)                                                                     \
+  FL_HIP(hipFuncSetAttribute((const void *)k_batch_yz_fwd<NCV>,                          \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[1])); \
+  FL_HIP(hipFuncSetAttribute((const void *)k_batch_zy_bwd<NCV>,                          \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[2]));
+  FL_BATCH_SIZES(FL_CASE)
+#undef FL_CASE
   return FLUTAS_OK;
 }
 
 int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *items,
-                       int nitems, size_t lds_bytes, hipStream_t s) {
+                       int nitems, size_t lds_bytes, int n, hipStream_t s) {
   if (nitems <= 0) return FLUTAS_OK;
   if (stage < BATCH_X_FWD || stage > BATCH_X_BWD) return fail(FLUTAS_EINVAL, "bad batch stage");
-  switch (stage) {
-    case BATCH_X_FWD:
-      hipLaunchKernelGGL(k_batch_x_fwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-      break;
-    case BATCH_YZ_FWD:
-      hipLaunchKernelGGL(k_batch_yz_fwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-      break;
-    case BATCH_ZY_BWD:
-      hipLaunchKernelGGL(k_batch_zy_bwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-      break;
-    default:
-      hipLaunchKernelGGL(k_batch_x_bwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-      break;
+  if (stage == BATCH_X_FWD) {
+    hipLaunchKernelGGL(k_batch_x_fwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
+  } else if (stage == BATCH_X_BWD) {
+    hipLaunchKernelGGL(k_batch_x_bwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
+  } else {
+    switch (n / 2) {
+#define FL_CASE(NCV)                                                                          \
+  case NCV:                                                                                   \
+    if (stage == BATCH_YZ_FWD)                                                                \
+      hipLaunchKernelGGL(k_batch_yz_fwd<NCV>, dim3(nitems),                                   \
+                         dim3(batch_slab_threads<NCV>()), lds_bytes, s, probs, items); \
+    else                                                                                      \
+      hipLaunchKernelGGL(k_batch_zy_bwd<NCV>, dim3(nitems),                                   \
+                         dim3(batch_slab_threads<NCV>()), lds_bytes, s, probs, items); \
+    break;
+      FL_BATCH_SIZES(FL_CASE)
+#undef FL_CASE
+      default:
+        return fail(FLUTAS_EINVAL, "bad batch line length");
+    }
   }
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;

@@ -121,12 +121,13 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 // of two tiles or slabs allows.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
-// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and more
-// (measured faster than 256 for the transposed loads of P4), 256 below
-// (faster for the short lines of batched problems).
+// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points
+// (measured faster than 256 for the transposed loads of P4), 256 otherwise
+// (faster for the short lines of batched problems; longer lines need more
+// than 128 registers per thread).
 template <int NC>
 constexpr int tile_threads() {
-  return 2 * NC >= 512 ? 512 : 256;
+  return 2 * NC == 512 ? 512 : 256;
 }
 
 // ---------------------------------------------------------------- P1 -----
@@ -694,15 +695,24 @@ constexpr int group_planes() {
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
 
-// Threads of a P2/P3 slab workgroup, as measured on 512-point y-lines: 512
-// threads (one column each, at most 128 registers, 4 waves per SIMD) for
-// the non-periodic passes, 256 for the periodic-z passes (whose cyclic
-// state does not fit 128 registers without spilling); 256 below 512
-// points.
+// Threads of a P2/P3 slab workgroup of a single problem.  512-point
+// y-lines: 512 threads (one column each, at most 128 registers, 4 waves per
+// SIMD) for the non-periodic passes, 256 for the periodic-z passes (whose
+// cyclic state does not fit 128 registers without spilling); 256 for other
+// lengths (a problem with few x-slabs has few workgroups, and more threads
+// shorten each one).
 template <typename T, int NC, bool PER>
 constexpr int slab_threads() {
   if (2 * NC != 512) return 256;
   return PER ? 256 : 512;
+}
+
+// Threads of a batched slab workgroup (lines of up to 256 points): one
+// column per thread in whole waves.  A batch has workgroups enough to fill
+// the GPU, so no wave should idle in the elimination.
+template <int NC>
+constexpr int batch_slab_threads() {
+  return ((2 * NC + 63) / 64) * 64;
 }
 
 }  // namespace flutas

@@ -160,7 +160,7 @@ size_t batch_lds_bytes(int stage, int n);
 // launching or capturing them)
 int batch_set_lds(const size_t lds_bytes[4]);
 int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *items,
-                       int nitems, size_t lds_bytes, hipStream_t s);
+                       int nitems, size_t lds_bytes, int n, hipStream_t s);
 
 // ---- refinement helpers ----------------------------------------------------
 struct OperatorDesc {

@@ -761,6 +761,15 @@ struct flutas_batch {
   BatchItem *d_items[4] = {nullptr, nullptr, nullptr, nullptr};
   int nitems[4] = {0, 0, 0, 0};
   size_t lds[4] = {0, 0, 0, 0};
+  // Each pass is launched once per line length (a segment of its item
+  // table), with the dynamic LDS that length needs: a launch sized for the
+  // largest problem would hold every small problem to its occupancy.
+  struct Segment {
+    int off, cnt;
+    size_t lds;
+    int n;  // line length of every item of the segment
+  };
+  std::vector<Segment> segs[4];
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -792,22 +801,41 @@ static int batch_upload(flutas_batch *b) {
 // graph, whose launch sizes it fixes.
 static int batch_items(flutas_batch *b) {
   std::vector<BatchItem> items[4];
-  for (int q = 0; q < b->nprob; ++q) {
+  // problems by line length of the pass (nx for the x passes, ny for the
+  // slab passes), longest first; within a length, the slab passes take the
+  // problems with the most planes per slab first
+  std::vector<int> ox(b->nprob), oy(b->nprob);
+  for (int q = 0; q < b->nprob; ++q) ox[q] = oy[q] = q;
+  std::stable_sort(ox.begin(), ox.end(), [&](int x, int y) { return b->probs[x].nx > b->probs[y].nx; });
+  std::stable_sort(oy.begin(), oy.end(), [&](int x, int y) {
+    const BatchProblem &px = b->probs[x], &py = b->probs[y];
+    if (px.ny != py.ny) return px.ny > py.ny;
+    return (long long)px.nz * px.ny > (long long)py.nz * py.ny;
+  });
+  for (int st = 0; st < 4; ++st) b->segs[st].clear();
+  auto segment = [&](int st, int n, int off) {
+    auto &v = b->segs[st];
+    if (v.empty() || v.back().n != n) v.push_back({off, 0, batch_lds_bytes(st, n), n});
+    v.back().cnt = (int)items[st].size() - v.back().off;
+  };
+  for (int q : ox) {
     const BatchProblem &p = b->probs[q];
     const int tj = batch_tile_lines(p.nx, p.tj_half);
     const int ntile = (p.ny + tj - 1) / tj;
+    const int off = (int)items[BATCH_X_FWD].size();
     for (int k = 0; k < p.nz; ++k)
       for (int t = 0; t < ntile; ++t) items[BATCH_X_FWD].push_back({q, t, k});
+    segment(BATCH_X_FWD, p.nx, off);
   }
-  std::vector<int> order(b->nprob);
-  for (int q = 0; q < b->nprob; ++q) order[q] = q;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return (long long)b->probs[x].nz * b->probs[x].ny > (long long)b->probs[y].nz * b->probs[y].ny;
-  });
-  for (int q : order)
+  for (int q : oy) {
+    const int off = (int)items[BATCH_YZ_FWD].size();
     for (int i = 0; i < b->probs[q].nx; ++i) items[BATCH_YZ_FWD].push_back({q, i, 0});
+    segment(BATCH_YZ_FWD, b->probs[q].ny, off);
+  }
   items[BATCH_ZY_BWD] = items[BATCH_YZ_FWD];
   items[BATCH_X_BWD] = items[BATCH_X_FWD];
+  b->segs[BATCH_ZY_BWD] = b->segs[BATCH_YZ_FWD];
+  b->segs[BATCH_X_BWD] = b->segs[BATCH_X_FWD];
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -825,7 +853,8 @@ static int batch_items(flutas_batch *b) {
 
 static int batch_enqueue(flutas_batch *b, hipStream_t s) {
   for (int st = 0; st < 4; ++st)
-    FL_TRY(launch_batch_stage(st, b->d_probs, b->d_items[st], b->nitems[st], b->lds[st], s));
+    for (const auto &sg : b->segs[st])
+      FL_TRY(launch_batch_stage(st, b->d_probs, b->d_items[st] + sg.off, sg.cnt, sg.lds, sg.n, s));
   return FLUTAS_OK;
 }
 

@@ -1,9 +1,10 @@
 """Batched solves (flutas_batch_*): every problem of an uneven, mixed-BC
-batch must equal its own single-problem FP64 solve bit for bit (the batched
-kernels run the same device bodies with the same tile shapes); after
-autotuning, whose halved tiles let the compiler contract the transform
-arithmetic differently, to 1e-13 relative max-norm, also after new fields
-are bound to the captured graph; one problem is also checked against the
+batch must equal its own single-problem FP64 solve to 1e-13 relative
+max-norm (the batched kernels run the same device bodies, with one thread
+per column in the slab passes where the single solve runs 256, and, after
+autotuning, halved tiles: the compiler may contract the transform
+arithmetic differently), also on a second replay and after new fields are
+bound to the captured graph; one problem is also checked against the
 oracle."""
 import numpy as np
 import pytest
@@ -60,11 +61,12 @@ def test_batch_matches_single_solves(api, oracle):
         p.zero()
     batch.set_fields(fs, ps)
     batch.solve()
-    for q, (p, r) in enumerate(zip(ps, ref)):
-        np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"problem {q}")
-    # a second replay of the captured graph gives the same
+    first = [p.to_host() for p in ps]
+    for q, (p, r) in enumerate(zip(first, ref)):
+        assert relmax(interior(p), interior(r)) <= 1e-13, f"problem {q}"
+    # a second replay of the captured graph gives the same bits
     batch.solve()
-    for q, (p, r) in enumerate(zip(ps, ref)):
+    for q, (p, r) in enumerate(zip(ps, first)):
         np.testing.assert_array_equal(interior(p.to_host()), interior(r), err_msg=f"replay {q}")
     # the oracle on the first problem
     ng, dli, dzci, dzfi, cbc = problems[0]

@@ -770,6 +770,13 @@ struct flutas_batch {
     int n;  // line length of every item of the segment
   };
   std::vector<Segment> segs[4];
+  // Segments of one pass are independent: the captured graph runs them on
+  // side streams in parallel (fork and join through events), so the
+  // workgroups of short-line problems fill the CUs that a long-line
+  // segment leaves free.  Default stream priority.
+  static constexpr int kSide = 4, kEvents = 16;
+  hipStream_t side[kSide] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[4][kEvents] = {};
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -780,6 +787,11 @@ static void batch_release(flutas_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (hipStream_t t : b->side)
+    if (t) (void)hipStreamDestroy(t);
+  for (auto &row : b->ev)
+    for (hipEvent_t e : row)
+      if (e) (void)hipEventDestroy(e);
   for (BatchItem *d : b->d_items)
     if (d) (void)hipFree(d);
   if (b->d_probs) (void)hipFree(b->d_probs);
@@ -852,9 +864,30 @@ static int batch_items(flutas_batch *b) {
 }
 
 static int batch_enqueue(flutas_batch *b, hipStream_t s) {
-  for (int st = 0; st < 4; ++st)
-    for (const auto &sg : b->segs[st])
-      FL_TRY(launch_batch_stage(st, b->d_probs, b->d_items[st] + sg.off, sg.cnt, sg.lds, sg.n, s));
+  for (int st = 0; st < 4; ++st) {
+    const auto &sg = b->segs[st];
+    auto launch = [&](size_t q, hipStream_t t) {
+      return launch_batch_stage(st, b->d_probs, b->d_items[st] + sg[q].off, sg[q].cnt, sg[q].lds,
+                                sg[q].n, t);
+    };
+    const size_t nseg = sg.size();
+    if (nseg <= 1 || nseg > (size_t)flutas_batch::kEvents - 1) {
+      for (size_t q = 0; q < nseg; ++q) FL_TRY(launch(q, s));
+      continue;
+    }
+    // fork: segment q > 0 on side stream (q-1) % kSide after the pass so far
+    hipEvent_t *e = b->ev[st];
+    FL_HIP(hipEventRecord(e[0], s));
+    for (size_t q = 1; q < nseg; ++q) {
+      hipStream_t t = b->side[(q - 1) % flutas_batch::kSide];
+      FL_HIP(hipStreamWaitEvent(t, e[0], 0));
+      FL_TRY(launch(q, t));
+      FL_HIP(hipEventRecord(e[q], t));
+    }
+    FL_TRY(launch(0, s));
+    // join: the next pass starts when every segment of this one has ended
+    for (size_t q = 1; q < nseg; ++q) FL_HIP(hipStreamWaitEvent(s, e[q], 0));
+  }
   return FLUTAS_OK;
 }
 
@@ -921,6 +954,13 @@ int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double 
   if (rc == FLUTAS_OK) rc = batch_set_lds(b->lds);
   if (rc == FLUTAS_OK && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(FLUTAS_EDEVICE, "hipStreamCreateWithFlags");
+  for (int t = 0; t < flutas_batch::kSide && rc == FLUTAS_OK; ++t)
+    if (hipStreamCreateWithFlags(&b->side[t], hipStreamNonBlocking) != hipSuccess)
+      rc = fail(FLUTAS_EDEVICE, "hipStreamCreateWithFlags");
+  for (int st = 0; st < 4 && rc == FLUTAS_OK; ++st)
+    for (int q = 0; q < flutas_batch::kEvents && rc == FLUTAS_OK; ++q)
+      if (hipEventCreateWithFlags(&b->ev[st][q], hipEventDisableTiming) != hipSuccess)
+        rc = fail(FLUTAS_EDEVICE, "hipEventCreateWithFlags");
   if (rc) {
     batch_release(b);
     delete b;

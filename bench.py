@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the pressure Poisson solve on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 A step is one solve of the 512^3 stretched-z channel (periodic x and y,
 Neumann z walls, Lx = 2 pi, Ly = pi, tanh stretching 1.5) on a seeded N(0,1)
@@ -209,7 +209,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg2_512_channel")
+    ap.add_argument("--workload", default="channel_512_stretched")
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-refine", action="store_true")
@@ -226,7 +226,7 @@ def main():
         sys.exit(2)
 
     from flutas_amd import api, workloads as wl
-    case = wl.CONFIGS[args.config]
+    case = wl.WORKLOADS[args.workload]
     nx, ny, nz = case.ng
     npts = case.npoints
     log(f"bench: {case.name} {case.ng} {case.cbc} stretch={case.alpha}")

@@ -37,15 +37,15 @@ class Case:
         return int(np.prod(self.ng))
 
 
-# The benchmark configurations (sizes and BCs of the five configs).
-CONFIGS = {
-    "cfg1_128_ppnn": Case("cfg1_128_ppnn", (128, 128, 128), (1.0, 1.0, 1.0), "PPPPNN", 0.0, 1),
-    "cfg2_512_channel": Case("cfg2_512_channel", (512, 512, 512),
-                             (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5, 2),
-    "cfg3_256x256x512_twophase": Case("cfg3_256x256x512_twophase", (256, 256, 512),
-                                      (1.0, 1.0, 2.0), "PPPPPP", 0.0, 3),
-    "cfg4_1024x1024x512_nnn": Case("cfg4_1024x1024x512_nnn", (1024, 1024, 512),
-                                   (1.0, 1.0, 1.0), "NNNNNN", 0.0, 4),
+# The benchmark workloads by name (the uneven batch is batch_problems()).
+WORKLOADS = {
+    "channel_128_uniform": Case("channel_128_uniform", (128, 128, 128), (1.0, 1.0, 1.0), "PPPPNN", 0.0, 1),
+    "channel_512_stretched": Case("channel_512_stretched", (512, 512, 512),
+                                  (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5, 2),
+    "twophase_256x256x512_periodic": Case("twophase_256x256x512_periodic", (256, 256, 512),
+                                          (1.0, 1.0, 2.0), "PPPPPP", 0.0, 3),
+    "neumann_1024x1024x512": Case("neumann_1024x1024x512", (1024, 1024, 512),
+                                  (1.0, 1.0, 1.0), "NNNNNN", 0.0, 4),
 }
 
 

@@ -1,15 +1,15 @@
-"""The measurement configurations at their real sizes on the GPU.
+"""The measurement workloads at their real sizes on the GPU.
 
-  config 1  128^3 PP/PP/NN: one projection step (fillps, updt_rhs_b,
-            solve, boundp, correc) against the same step through the
-            oracle; divergence after the correction.
-  config 3  256x256x512 triply periodic two-phase step (pres_tw_src with
-            the constant-coefficient splitting, fillps, solve, correc)
-            against the oracle, plain FP64 and FP32 + refinement.
-  config 4  1024x1024x512 NN/NN/NN (4 GiB per FP64 field): too large for
-            the oracle in a test, checked through size-independent
-            properties: the relative residual of the FP64 solve, the
-            refined solve reaching 1e-10, and the agreement of the two.
+  channel_128_uniform            128^3 PP/PP/NN: one projection step
+      (fillps, updt_rhs_b, solve, boundp, correc) against the same step
+      through the oracle; divergence after the correction.
+  twophase_256x256x512_periodic  triply periodic two-phase step
+      (pres_tw_src with the constant-coefficient splitting, fillps, solve,
+      correc) against the oracle, plain FP64 and FP32 + refinement.
+  neumann_1024x1024x512          NN/NN/NN (4 GiB per FP64 field): too
+      large for the oracle in a test, checked through size-independent
+      properties: the relative residual of the FP64 solve, the refined
+      solve reaching 1e-10, and the agreement of the two.
 """
 import numpy as np
 import pytest
@@ -41,14 +41,14 @@ def _velocity(case, nh_u=1, scale=1.0):
     return _halos(case, *wl.velocity_uniform(case, scale=scale, nh_u=nh_u))
 
 
-def test_config1_projection_step(api, oracle):
+def test_channel_128_projection_step(api, oracle):
     from flutas_amd import workloads as wl
-    case = wl.CONFIGS["cfg1_128_ppnn"]
+    case = wl.WORKLOADS["channel_128_uniform"]
     nx, ny, nz = case.ng
     dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
     dli = case.dli
     u, v, w = _velocity(case)
-    print(f"config 1 {case.ng} {case.cbc}", flush=True)
+    print(f"{case.name} {case.ng} {case.cbc}", flush=True)
     D = api.DeviceArray.from_host
     ud, vd, wd, dzcid, dzfid = D(u), D(v), D(w), D(dzci), D(dzfi)
     plan = api.PoissonPlan(case.ng, dli, dzci, dzfi, case.cbc)
@@ -82,9 +82,9 @@ def test_config1_projection_step(api, oracle):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_config3_two_phase_step(api, oracle, precision):
+def test_twophase_256x256x512_step(api, oracle, precision):
     from flutas_amd import workloads as wl
-    case = wl.CONFIGS["cfg3_256x256x512_twophase"]
+    case = wl.WORKLOADS["twophase_256x256x512_periodic"]
     nx, ny, nz = case.ng
     dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
     dli = case.dli
@@ -95,7 +95,7 @@ def test_config3_two_phase_step(api, oracle, precision):
     rho = wl.sphere_density(case)
     rho0 = float(rho.min())
     dt = 1e-3
-    print(f"config 3 {case.ng} precision {precision}", flush=True)
+    print(f"{case.name} {case.ng} precision {precision}", flush=True)
     D = api.DeviceArray.from_host
     ud, vd, wd, pd, poldd, rhod = D(u), D(v), D(w), D(p), D(pold), D(rho)
     dzcid, dzfid = D(dzci), D(dzfi)
@@ -131,11 +131,11 @@ def test_config3_two_phase_step(api, oracle, precision):
         assert relmax(gauge_free(got), gauge_free(interior(ref))) <= 1e-8
 
 
-def test_config4_large_nn(api):
+def test_neumann_1024x1024x512_solve(api):
     from flutas_amd import workloads as wl
-    case = wl.CONFIGS["cfg4_1024x1024x512_nnn"]
+    case = wl.WORKLOADS["neumann_1024x1024x512"]
     dzc, dzf, dzci, dzfi = wl.z_grid(api, case)
-    print(f"config 4 {case.ng} {case.cbc}", flush=True)
+    print(f"{case.name} {case.ng} {case.cbc}", flush=True)
     fd = api.DeviceArray.from_host(wl.rhs_normal(case, dzf))
     p64 = api.DeviceArray(fd.shape)
     p32 = api.DeviceArray(fd.shape)

@@ -1,14 +1,16 @@
 // Batched four-pass pipeline: many independent FP64 Poisson problems of
-// uneven sizes and mixed boundary conditions, one launch per pass.
+// uneven sizes and mixed boundary conditions.
 //
 // Small problems cannot fill the GPU on their own (a slab pass has one
-// workgroup per x-slab, e.g. 64 for a 64-wide grid), so every pass of a
-// batch is a single launch whose workgroups are the tiles or slabs of all
-// problems.  A workgroup reads its (problem, tile) pair from the item table
-// and runs the body compiled for that problem's line length (switch on the
-// half length), with the tile in dynamic LDS sized for the largest problem.
-// Each problem also carries two tuned choices, the halved x tile and the
-// halved slab plane group, which the batch autotuner sets per size.
+// workgroup per x-slab, e.g. 64 for a 64-wide grid), so each pass of a
+// batch is one launch per line length present, whose workgroups are the
+// tiles or slabs of all problems of that length.  A workgroup reads its
+// (problem, tile) pair from the item table and runs the body compiled for
+// that problem's line length (tile passes: a switch on the half length;
+// slab passes: one kernel instance per length), with the tile or slab in
+// dynamic LDS sized for that length.  Each problem also carries two tuned
+// choices, the halved x tile and the halved slab plane group, which the
+// batch autotuner sets per size.
 #include "fused_body.hpp"
 
 namespace flutas {

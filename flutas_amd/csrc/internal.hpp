@@ -127,9 +127,9 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
                   int periodic, hipStream_t s);
 
 // ---- batched fused pipeline (many independent FP64 problems) ------------
-// One launch per stage covers every problem of a batch; a workgroup reads
-// its (problem, tile) from the item table and runs the body compiled for
-// that problem's line lengths.
+// One launch per stage and line length covers every problem of that length;
+// a workgroup reads its (problem, tile) from the item table and runs the
+// body compiled for that problem's line lengths.
 struct BatchProblem {
   int nx, ny, nz;
   int ncx, ncy;                    // nx / 2, ny / 2 (compiled plan selector)
@@ -156,9 +156,11 @@ int batch_tile_lines(int nx, int half);
 int batch_group_planes(int ny, int half);
 // dynamic LDS bytes of a stage for line length n (the largest variant)
 size_t batch_lds_bytes(int stage, int n);
-// raises the dynamic-LDS limit of the four batched kernels (call before
+// raises the dynamic-LDS limit of the batched kernels (call before
 // launching or capturing them)
 int batch_set_lds(const size_t lds_bytes[4]);
+// one launch of a stage over nitems items whose line length (nx for the x
+// stages, ny for the slab stages) is n
 int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *items,
                        int nitems, size_t lds_bytes, int n, hipStream_t s);
 

@@ -8,7 +8,9 @@
 // A field file is the global array, FP64, Fortran order, no halo and no
 // header (what MPI-IO writes at one rank); its size is checked against the
 // grid on reading, as the reference does.  The interior of a device field
-// with halo h is moved through a dense device buffer and a copy kernel.
+// with halo h is moved through a dense device buffer and a copy kernel; a
+// field in plain host memory (the explicit-shape arrays the reference's
+// load receives) is read or written directly, with no device work.
 #include "internal.hpp"
 
 #include <cmath>
@@ -48,19 +50,37 @@ int check_args(char io, const char *filename, const int n[3], int h) {
   return FLUTAS_OK;
 }
 
+// dense (n0,n1,n2) <-> interior of a halo-h host field
+void host_move(double *dense, double *fld, const int n[3], int h, int to_field) {
+  const long long s0 = n[0] + 2 * h, s1 = (long long)s0 * (n[1] + 2 * h);
+  for (int k = 0; k < n[2]; ++k)
+    for (int j = 0; j < n[1]; ++j) {
+      double *d = dense + (long long)n[0] * (j + (long long)n[1] * k);
+      double *f = fld + h + s0 * (j + h) + s1 * (k + h);
+      if (to_field) std::memcpy(f, d, sizeof(double) * n[0]);
+      else std::memcpy(d, f, sizeof(double) * n[0]);
+    }
+}
+
 // Reads (io 'r') or writes (io 'w') `count` fields of n points each, then
-// `nextra` doubles, at the given device fields.
+// `nextra` doubles, at the given device or host fields.
 int field_file(char io, const char *filename, const int n[3], const int *halo,
                double *const *fld, int count, double *extra, int nextra) {
   const size_t npts = (size_t)n[0] * n[1] * n[2];
   const size_t want = (npts * count + nextra) * sizeof(double);
   std::vector<double> host(npts);
+  std::vector<int> on_dev(count, 0);
+  bool any_dev = false;
+  for (int f = 0; f < count; ++f) {
+    FL_TRY(flutas_pointer_is_device(fld[f], &on_dev[f]));
+    any_dev = any_dev || on_dev[f];
+  }
   double *dense = nullptr;
-  FL_HIP(hipMalloc((void **)&dense, npts * sizeof(double)));
+  if (any_dev) FL_HIP(hipMalloc((void **)&dense, npts * sizeof(double)));
   int rc = FLUTAS_OK;
   FILE *fh = std::fopen(filename, io == 'r' ? "rb" : "wb");
   if (!fh) {
-    (void)hipFree(dense);
+    if (dense) (void)hipFree(dense);
     return fail(FLUTAS_EINVAL, std::string("cannot open ") + filename);
   }
   if (io == 'r') {
@@ -78,17 +98,25 @@ int field_file(char io, const char *filename, const int n[3], const int *halo,
         rc = fail(FLUTAS_EINVAL, "short read");
         break;
       }
+      if (!on_dev[f]) {
+        host_move(host.data(), fld[f], n, halo[f], 1);
+        continue;
+      }
       if (hipMemcpy(dense, host.data(), npts * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(FLUTAS_EDEVICE, "hipMemcpy");
         break;
       }
       rc = move_interior(dense, fld[f], n, halo[f], 1);
     } else {
-      rc = move_interior(dense, fld[f], n, halo[f], 0);
-      if (rc) break;
-      if (hipMemcpy(host.data(), dense, npts * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
-        rc = fail(FLUTAS_EDEVICE, "hipMemcpy");
-        break;
+      if (!on_dev[f]) {
+        host_move(host.data(), fld[f], n, halo[f], 0);
+      } else {
+        rc = move_interior(dense, fld[f], n, halo[f], 0);
+        if (rc) break;
+        if (hipMemcpy(host.data(), dense, npts * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+          rc = fail(FLUTAS_EDEVICE, "hipMemcpy");
+          break;
+        }
       }
       if (std::fwrite(host.data(), sizeof(double), npts, fh) != npts) rc = fail(FLUTAS_EINVAL, "short write");
     }
@@ -101,7 +129,7 @@ int field_file(char io, const char *filename, const int n[3], const int *halo,
     }
   }
   std::fclose(fh);
-  (void)hipFree(dense);
+  if (dense) (void)hipFree(dense);
   return rc;
 }
 
@@ -120,10 +148,18 @@ int flutas_load(char io, const char *filename, const int n[3], int halo, double 
 
 int flutas_load_ns(char io, const char *filename, const int n[3], int halo_u, double *u,
                    double *v, double *w, double *p, double *time, double *istep) {
-  FL_TRY(check_args(io, filename, n, halo_u));
+  const int halos[4] = {halo_u, halo_u, halo_u, 1};
+  return flutas_load_ns_halos(io, filename, n, halos, u, v, w, p, time, istep);
+}
+
+int flutas_load_ns_halos(char io, const char *filename, const int n[3], const int halo[4],
+                         double *u, double *v, double *w, double *p, double *time,
+                         double *istep) {
+  if (!halo) return fail(FLUTAS_EINVAL, "null pointer");
+  for (int q = 0; q < 4; ++q) FL_TRY(check_args(io, filename, n, halo[q]));
   if (!u || !v || !w || !p || !time || !istep) return fail(FLUTAS_EINVAL, "null pointer");
   double *flds[4] = {u, v, w, p};
-  const int halos[4] = {halo_u, halo_u, halo_u, 1};
+  const int *halos = halo;
   double info[2] = {*time, *istep};
   FL_TRY(field_file(io, filename, n, halos, flds, 4, info, 2));
   if (io == 'r') {

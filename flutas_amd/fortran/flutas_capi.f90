@@ -162,6 +162,36 @@ module flutas_capi
       real(c_double) :: divtot, divmax
       integer(c_int) :: rc
     end function
+    ! restart files (src/load.f90)
+    function flutas_load(io, filename, n, halo, fld) bind(C, name="flutas_load") result(rc)
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char), value :: io
+      character(kind=c_char), dimension(*) :: filename
+      integer(c_int), dimension(3) :: n
+      integer(c_int), value :: halo
+      type(c_ptr), value :: fld
+      integer(c_int) :: rc
+    end function
+    function flutas_load_ns_halos(io, filename, n, halo, u, v, w, p, time, istep) &
+        bind(C, name="flutas_load_ns_halos") result(rc)
+      import :: c_int, c_char, c_ptr, c_double
+      character(kind=c_char), value :: io
+      character(kind=c_char), dimension(*) :: filename
+      integer(c_int), dimension(3) :: n
+      integer(c_int), dimension(4) :: halo
+      type(c_ptr), value :: u, v, w, p
+      real(c_double) :: time, istep
+      integer(c_int) :: rc
+    end function
+    function flutas_load_scalar(io, filename, time, istep, dto) bind(C, name="flutas_load_scalar") &
+        result(rc)
+      import :: c_int, c_char, c_double
+      character(kind=c_char), value :: io
+      character(kind=c_char), dimension(*) :: filename
+      real(c_double) :: time, dto
+      integer(c_int) :: istep
+      integer(c_int) :: rc
+    end function
     function flutas_initgrid(n, gr, lz, nh_d, dzc, dzf, zc, zf) bind(C, name="flutas_initgrid") result(rc)
       import :: c_int, c_double
       integer(c_int), value :: n, nh_d

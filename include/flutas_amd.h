@@ -165,14 +165,20 @@ int flutas_batch_destroy(flutas_batch *b);
 /* load (src/load.f90:21-90): io 'r' reads, 'w' writes the file `filename`
  * holding the global n[0] x n[1] x n[2] FP64 field in Fortran order (no
  * header, no halo; the reference's MPI-IO layout at one rank) from / into
- * the interior of the device field fld with halo `halo`.  A file of the
- * wrong size is rejected (EINVAL), as the reference aborts. */
+ * the interior of the field fld with halo `halo` (a device field, or plain
+ * host memory, then with no device work).  A file of the wrong size is
+ * rejected (EINVAL), as the reference aborts. */
 int flutas_load(char io, const char *filename, const int n[3], int halo, double *fld);
 /* load_ns (src/load.f90:132-204), the CaNS layout: u, v, w (halo
  * halo_u), p (halo 1), then the two doubles time, istep. */
 int flutas_load_ns(char io, const char *filename, const int n[3], int halo_u,
                    double *u, double *v, double *w, double *p, double *time,
                    double *istep);
+/* The same with one halo width per field (u, v, w, p): 0 for the dense
+ * explicit-shape arrays load_ns itself receives (src/load.f90:141-142). */
+int flutas_load_ns_halos(char io, const char *filename, const int n[3],
+                         const int halo[4], double *u, double *v, double *w,
+                         double *p, double *time, double *istep);
 /* load_scalar (src/load.f90:92-130): the text file "time dto istep",
  * written with the format (2E15.7, 1I9.8).  Host values only. */
 int flutas_load_scalar(char io, const char *filename, double *time, int *istep,

@@ -103,3 +103,30 @@ def test_load_scalar_reference_format(tmp_path):
     assert api.load_scalar("r", f) == (2.5, 7, 1e-4)
     with pytest.raises(api.FlutasError):
         api.load_scalar("r", tmp_path / "missing.out")
+
+
+def test_fortran_mod_load_round_trip(tmp_path):
+    """The drop-in mod_load (src/load.f90: load, load_ns, load_scalar,
+    cmpt_it_chkpt) on host arrays, called as the reference's apps call it
+    (interior sections of halo arrays): round trips inside the Fortran demo,
+    and the files it writes hold the reference's layouts (raw FP64 global
+    arrays in Fortran order; u, v, w, p then time, istep)."""
+    import subprocess
+    demo = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "restart_demo")
+    if not os.path.exists(demo):
+        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
+    r = subprocess.run([demo, str(tmp_path) + "/"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "restart_demo ok 3" in r.stdout
+    n = (6, 5, 4)
+    i, j, k = np.meshgrid(*(np.arange(1, m + 1) for m in n), indexing="ij")
+    u = i + 10.0 * j + 100.0 * k + 0.25
+    got = np.fromfile(tmp_path / "fldu.bin", dtype=np.float64).reshape(n, order="F")
+    np.testing.assert_array_equal(got, u)
+    raw = np.fromfile(tmp_path / "fld.bin", dtype=np.float64)
+    npts = u.size
+    assert raw.size == 4 * npts + 2
+    for q, ref in enumerate((u, -u, 0.5 * u, 1.0 / (1.0 + u))):
+        np.testing.assert_array_equal(raw[q * npts:(q + 1) * npts].reshape(n, order="F"), ref)
+    assert tuple(raw[-2:]) == (1.5, 42.0)
+    assert (tmp_path / "scalar.out").read_text() == "  0.1250000E+00  0.1000000E-02 00001234\n"

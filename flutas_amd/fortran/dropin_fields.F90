@@ -178,6 +178,8 @@ contains
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dz = stage_in(5, c_loc(dzfi), nbytes(size(dzfi)), .true.)
     call flutas_check(flutas_chkdiv(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dz,du,dv,dw,divtot,divmax), 'chkdiv')
+    ! the report the reference prints (src/chkdiv.f90:66; one rank)
+    print*, 'Total divergence = ', divtot, '| Maximum divergence = ', divmax
   end subroutine chkdiv
 end module mod_chkdiv
 !

@@ -66,6 +66,8 @@ def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
     r = subprocess.run([DEMO, str(fin), str(fout), mode], capture_output=True, text=True,
                        timeout=100)
     assert r.returncode == 0, r.stdout + r.stderr
+    # chkdiv reports as the reference does (src/chkdiv.f90:66)
+    assert "Total divergence = " in r.stdout and "| Maximum divergence = " in r.stdout
 
     hp = (nx + 2, ny + 2, nz + 2)
     hu = u.shape

@@ -63,6 +63,15 @@ def test_invalid_arguments_rejected_without_device():
     with pytest.raises(api.FlutasError) as e:
         api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", precision=16)
     assert e.value.code == api.FLUTAS_EINVAL
+    # odd sizes (src/sanity.f90:155) and a periodic z shorter than the
+    # cyclic closure needs
+    with pytest.raises(api.FlutasError) as e:
+        api.PoissonPlan((8, 7, 8), (1, 1, 1), dz, dz, "PPPPNN")
+    assert e.value.code == api.FLUTAS_EINVAL
+    dz2 = np.ones(4)
+    with pytest.raises(api.FlutasError) as e:
+        api.PoissonPlan((8, 8, 2), (1, 1, 1), dz2, dz2, "PPPPPP")
+    assert e.value.code == api.FLUTAS_EINVAL
 
 
 def test_fortran_dropin_fails_loudly_without_gpu(tmp_path):

@@ -88,6 +88,12 @@ SOLVE_CASES = [
     ((512, 64, 16), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
     ((48, 512, 12), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
     ((64, 512, 10), (1.0, 1.0, 1.0), "DDDDPP", 0.0),
+    # edge sizes: the smallest grid, nz = 2, 768- and 1024-point lines
+    ((2, 2, 2), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((4, 2, 6), (1.0, 1.0, 1.0), "PPPPDD", 0.0),
+    ((6, 4, 2), (1.0, 1.0, 1.0), "PPNNNN", 0.0),
+    ((768, 32, 6), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((32, 1024, 4), (1.0, 1.0, 1.0), "NNPPNN", 0.0),
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
@@ -253,7 +259,7 @@ def test_boundp_updt_rhs_b_bit_identical(api, oracle, cbc):
 
 
 @pytest.mark.parametrize("alpha", [0.0, 1.5])
-def test_projection_step_config1_shape(api, oracle, alpha):
+def test_projection_step_channel_64(api, oracle, alpha):
     """The step of main__two_phase_inc_isot.f90:656-716 on a 64^3 PP/PP/NN
     channel: fillps -> updt_rhs_b -> solve -> boundp -> correc -> chkdiv."""
     ng = (64, 64, 64)

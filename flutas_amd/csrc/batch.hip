@@ -6,9 +6,8 @@
 // batch is one launch per line length present, whose workgroups are the
 // tiles or slabs of all problems of that length.  A workgroup reads its
 // (problem, tile) pair from the item table and runs the body compiled for
-// that problem's line length (tile passes: a switch on the half length;
-// slab passes: one kernel instance per length), with the tile or slab in
-// dynamic LDS sized for that length.  Each problem also carries two tuned
+// that line length (one kernel instance per length and pass), with the
+// tile or slab in dynamic LDS sized for that length.  Each problem also carries two tuned
 // choices, the halved x tile and the halved slab plane group, which the
 // batch autotuner sets per size.
 #include "fused_body.hpp"
@@ -35,57 +34,43 @@ constexpr int gp_variant(int half) {
   return half ? (f > 1 ? f / 2 : 1) : f;
 }
 
+// The tile passes are launched once per x-line length (every item of a
+// launch has nx = 2*NC), so each length gets a kernel of its own registers.
+template <int NC>
 __global__ void __launch_bounds__(kBatchThreads)
     k_batch_x_fwd(const BatchProblem *__restrict__ P, const BatchItem *__restrict__ items) {
   extern __shared__ __align__(16) double dyn[];
   const BatchItem w = items[blockIdx.x];
   const BatchProblem &q = P[w.prob];
-  switch (q.ncx) {
-#define FL_CASE(NCV)                                                                       \
-  case NCV:                                                                                \
-    if (q.tj_half) {                                                                       \
-      constexpr int TJ = tj_variant<NCV>(1);                                               \
-      x_fwd_tile<double, double, NCV, TJ, kBatchThreads>(                                  \
-          q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,        \
-          dyn + TJ * line_stride<double, 2 * NCV>(), w.u * TJ, w.v);                       \
-    } else {                                                                               \
-      constexpr int TJ = tj_variant<NCV>(0);                                               \
-      x_fwd_tile<double, double, NCV, TJ, kBatchThreads>(                                  \
-          q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,        \
-          dyn + TJ * line_stride<double, 2 * NCV>(), w.u * TJ, w.v);                       \
-    }                                                                                      \
-    break;
-    FL_BATCH_SIZES(FL_CASE)
-#undef FL_CASE
-    default:
-      break;
+  if (q.tj_half) {
+    constexpr int TJ = tj_variant<NC>(1);
+    x_fwd_tile<double, double, NC, TJ, kBatchThreads>(
+        q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,
+        dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
+  } else {
+    constexpr int TJ = tj_variant<NC>(0);
+    x_fwd_tile<double, double, NC, TJ, kBatchThreads>(
+        q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,
+        dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   }
 }
 
+template <int NC>
 __global__ void __launch_bounds__(kBatchThreads)
     k_batch_x_bwd(const BatchProblem *__restrict__ P, const BatchItem *__restrict__ items) {
   extern __shared__ __align__(16) double dyn[];
   const BatchItem w = items[blockIdx.x];
   const BatchProblem &q = P[w.prob];
-  switch (q.ncx) {
-#define FL_CASE(NCV)                                                                       \
-  case NCV:                                                                                \
-    if (q.tj_half) {                                                                       \
-      constexpr int TJ = tj_variant<NCV>(1);                                               \
-      x_bwd_tile<double, double, NCV, TJ, kBatchThreads, false>(                           \
-          q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,        \
-          dyn + TJ * line_stride<double, 2 * NCV>(), w.u * TJ, w.v);                       \
-    } else {                                                                               \
-      constexpr int TJ = tj_variant<NCV>(0);                                               \
-      x_bwd_tile<double, double, NCV, TJ, kBatchThreads, false>(                           \
-          q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,        \
-          dyn + TJ * line_stride<double, 2 * NCV>(), w.u * TJ, w.v);                       \
-    }                                                                                      \
-    break;
-    FL_BATCH_SIZES(FL_CASE)
-#undef FL_CASE
-    default:
-      break;
+  if (q.tj_half) {
+    constexpr int TJ = tj_variant<NC>(1);
+    x_bwd_tile<double, double, NC, TJ, kBatchThreads, false>(
+        q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,
+        dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
+  } else {
+    constexpr int TJ = tj_variant<NC>(0);
+    x_bwd_tile<double, double, NC, TJ, kBatchThreads, false>(
+        q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,
+        dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   }
 }
 
@@ -190,11 +175,11 @@ size_t batch_lds_bytes(int stage, int n) {
 }
 
 int batch_set_lds(const size_t lds_bytes[4]) {
-  FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_fwd,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[0]));
-  FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_bwd,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[3]));
 #define FL_CASE(NCV)                                                                     \
+  FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_fwd<NCV>,                           \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[0])); \
+  FL_HIP(hipFuncSetAttribute((const void *)k_batch_x_bwd<NCV>,                           \
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[3])); \
   FL_HIP(hipFuncSetAttribute((const void *)k_batch_yz_fwd<NCV>,                          \
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes[1])); \
   FL_HIP(hipFuncSetAttribute((const void *)k_batch_zy_bwd<NCV>,                          \
@@ -208,15 +193,17 @@ int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *it
                        int nitems, size_t lds_bytes, int n, hipStream_t s) {
   if (nitems <= 0) return FLUTAS_OK;
   if (stage < BATCH_X_FWD || stage > BATCH_X_BWD) return fail(FLUTAS_EINVAL, "bad batch stage");
-  if (stage == BATCH_X_FWD) {
-    hipLaunchKernelGGL(k_batch_x_fwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-  } else if (stage == BATCH_X_BWD) {
-    hipLaunchKernelGGL(k_batch_x_bwd, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, probs, items);
-  } else {
+  {
     switch (n / 2) {
 #define FL_CASE(NCV)                                                                          \
   case NCV:                                                                                   \
-    if (stage == BATCH_YZ_FWD)                                                                \
+    if (stage == BATCH_X_FWD)                                                                 \
+      hipLaunchKernelGGL(k_batch_x_fwd<NCV>, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, \
+                         probs, items);                                                       \
+    else if (stage == BATCH_X_BWD)                                                            \
+      hipLaunchKernelGGL(k_batch_x_bwd<NCV>, dim3(nitems), dim3(kBatchThreads), lds_bytes, s, \
+                         probs, items);                                                       \
+    else if (stage == BATCH_YZ_FWD)                                                           \
       hipLaunchKernelGGL(k_batch_yz_fwd<NCV>, dim3(nitems),                                   \
                          dim3(batch_slab_threads<NCV>()), lds_bytes, s, probs, items); \
     else                                                                                      \

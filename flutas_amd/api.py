@@ -460,6 +460,15 @@ def fft_lines(kind, data, n, howmany, stride=1, dist=None):
                                 C.c_longlong(stride), C.c_longlong(dist), _dptr(data)))
 
 
+def fft_guru(kind, data, n, stride, howmany1, dist1, howmany2=1, dist2=0):
+    """fft over the two-level batch of fftini's guru plans
+    (src/fft.f90:75-86, :113-124)."""
+    k = R2R_KIND[kind] if isinstance(kind, str) else int(kind)
+    _check(lib.flutas_fft_guru(C.c_int(k), C.c_int(n), C.c_longlong(stride),
+                               C.c_int(howmany1), C.c_longlong(dist1),
+                               C.c_int(howmany2), C.c_longlong(dist2), _dptr(data)))
+
+
 def initgrid(n, gr, lz, nh_d=1):
     """src/initgrid.f90:17-97; returns dzc, dzf, zc, zf over (1-nh_d:n+nh_d)."""
     arrs = [np.zeros(n + 2 * nh_d) for _ in range(4)]

@@ -642,16 +642,22 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage) {
 
 int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data) {
-  if (!data || howmany < 1 || stride < 1) return fail(FLUTAS_EINVAL, "bad fft_lines arguments");
+  return flutas_fft_guru(kind, n, stride, howmany, dist, 1, 0, data);
+}
+
+int flutas_fft_guru(int kind, int n, long long stride, int howmany1, long long dist1,
+                    int howmany2, long long dist2, double *data) {
+  if (!data || howmany1 < 1 || howmany2 < 1 || howmany2 > 65535 || stride < 1 || n < 2)
+    return fail(FLUTAS_EINVAL, "bad fft arguments");
   if (kind != K_R2HC && kind != K_HC2R && kind != K_REDFT01 && kind != K_REDFT10 &&
       kind != K_RODFT01 && kind != K_RODFT10 && kind != K_REDFT11 && kind != K_RODFT11)
     return fail(FLUTAS_EUNSUPPORTED, "r2r kind not built");
   LineTables tab;
   FL_TRY(tab.build(n));
   LineGeom g;
-  g.n = n; g.L1 = howmany; g.L2 = 1;
-  g.is1 = dist; g.is2 = 0; g.ise = stride;
-  g.os1 = dist; g.os2 = 0; g.ose = stride;
+  g.n = n; g.L1 = howmany1; g.L2 = howmany2;
+  g.is1 = dist1; g.is2 = dist2; g.ise = stride;
+  g.os1 = dist1; g.os2 = dist2; g.ose = stride;
   int rc = launch_lines<double, double, double>(tab, kind, data, data, g, 1.0, current_stream());
   hipError_t e = hipStreamSynchronize(current_stream());
   tab.release();

@@ -5,7 +5,11 @@
 !               rank), fills lambdaxy, a, b, c, normfft and rhsb{x,y,z}
 !               exactly as the reference does, and keeps the plan handle
 !               in arrplan(1,1) (the other three entries are null).
-!   fftend      src/fft.f90:159-179        destroys the plan.
+!   fftini      src/fft.f90:24-157         the x and y r2r plans and normfft
+!   fft         src/fft.f90:181-193        (find_fft's table, src/fft.f90:233-291)
+!               on their own, for callers of the transforms; executed on
+!               the GPU.  Face-centred NN/DD (REDFT00/RODFT00) stop.
+!   fftend      src/fft.f90:159-179        destroys either kind of arrplan.
 !   solver_gpu  src/solver_gpu.f90:31-47   solves in place on the interior
 !   solver_cpu  src/solver_cpu.f90:20-34   of p; the transforms, eigenvalues
 !               and z coefficients come from the plan, so lambdaxy, a, b,
@@ -77,14 +81,154 @@ end module mod_initsolver
 module mod_fft
   use, intrinsic :: iso_c_binding
   use flutas_capi
-  use flutas_stage, only: stage_release
+  use flutas_stage, only: stage_in, stage_out, stage_release
   implicit none
   private
-  public :: fftend
+  public :: fftini, fftend, fft
+  integer, parameter :: rp = c_double
+  ! FFTW's r2r kind numbers (src/fftw.f90:38-61)
+  integer(c_int), parameter :: R2HC = 0, HC2R = 1, REDFT00 = 3, REDFT01 = 4, REDFT10 = 5, &
+                               REDFT11 = 6, RODFT00 = 7, RODFT01 = 8, RODFT10 = 9, RODFT11 = 10
+  ! One r2r plan of fftini: lines of length n, element stride `stride`,
+  ! howmany1 lines at distance dist1 repeated howmany2 times at distance
+  ! dist2, over an array of `elems` elements.
+  type :: line_plan
+    integer(c_int) :: kind, n, howmany1, howmany2
+    integer(c_long_long) :: stride, dist1, dist2
+    integer(c_size_t) :: elems
+  end type line_plan
+  ! the plans fftini made and fftend has not yet freed
+  type(c_ptr), allocatable, save :: issued(:)
 contains
+  ! Transform kinds and normalisation of one direction (find_fft,
+  ! src/fft.f90:233-291).  The face-centred NN and DD kinds (REDFT00,
+  ! RODFT00) are not built and stop the program.
+  subroutine find_fft(bc, c_or_f, kind_fwd, kind_bwd, norm)
+    character(len=1), intent(in ), dimension(0:1) :: bc
+    character(len=1), intent(in )                 :: c_or_f
+    integer(c_int)  , intent(out)                 :: kind_fwd, kind_bwd
+    real(rp)        , intent(out), dimension(2)   :: norm
+    character(len=2) :: pair
+    pair = bc(0)//bc(1)
+    if (c_or_f == 'c') then
+      select case (pair)
+      case ('PP'); kind_fwd = R2HC;    kind_bwd = HC2R;    norm = [1.0_rp, 0.0_rp]
+      case ('NN'); kind_fwd = REDFT10; kind_bwd = REDFT01; norm = [2.0_rp, 0.0_rp]
+      case ('DD'); kind_fwd = RODFT10; kind_bwd = RODFT01; norm = [2.0_rp, 0.0_rp]
+      case ('ND'); kind_fwd = REDFT11; kind_bwd = REDFT11; norm = [2.0_rp, 0.0_rp]
+      case ('DN'); kind_fwd = RODFT11; kind_bwd = RODFT11; norm = [2.0_rp, 0.0_rp]
+      case default; error stop 'flutas_amd fftini: unknown boundary-condition pair'
+      end select
+    else if (c_or_f == 'f') then
+      select case (pair)
+      case ('PP'); kind_fwd = R2HC;    kind_bwd = HC2R;    norm = [1.0_rp, 0.0_rp]
+      case ('ND'); kind_fwd = REDFT10; kind_bwd = REDFT01; norm = [2.0_rp, 0.0_rp]
+      case ('DN'); kind_fwd = RODFT01; kind_bwd = RODFT10; norm = [2.0_rp, 0.0_rp]
+      case ('NN', 'DD'); error stop 'flutas_amd fftini: face-centred NN/DD (REDFT00/RODFT00) is not built'
+      case default; error stop 'flutas_amd fftini: unknown boundary-condition pair'
+      end select
+    else
+      error stop 'flutas_amd fftini: c_or_f must be c or f'
+    end if
+  end subroutine find_fft
+  !
+  function new_plan(kind, n, stride, howmany1, dist1, howmany2, dist2, elems) result(cp)
+    integer(c_int), intent(in) :: kind
+    integer, intent(in) :: n, stride, howmany1, dist1, howmany2, dist2
+    integer(c_size_t), intent(in) :: elems
+    type(c_ptr) :: cp
+    type(line_plan), pointer :: lp
+    type(c_ptr), allocatable :: grown(:)
+    allocate(lp)
+    lp = line_plan(kind, int(n, c_int), int(howmany1, c_int), int(howmany2, c_int), &
+                   int(stride, c_long_long), int(dist1, c_long_long), int(dist2, c_long_long), elems)
+    cp = c_loc(lp)
+    if (.not. allocated(issued)) allocate(issued(0))
+    allocate(grown(size(issued) + 1))
+    grown(1:size(issued)) = issued
+    grown(size(grown)) = cp
+    call move_alloc(grown, issued)
+  end function new_plan
+  !
+  ! index in `issued` of a plan made by fftini, 0 if it is none
+  integer function plan_index(cp)
+    type(c_ptr), intent(in) :: cp
+    integer :: q
+    plan_index = 0
+    if (.not. allocated(issued)) return
+    do q = 1, size(issued)
+      if (c_associated(issued(q), cp)) then
+        plan_index = q
+        return
+      end if
+    end do
+  end function plan_index
+  !
+  ! fftini (src/fft.f90:24-157): the forward and backward r2r plans of x
+  ! (arrplan(:,1), over n_x arrays) and y (arrplan(:,2), over n_y arrays) and
+  ! normfft = 1 / prod(norm(1) * (n + norm(2))).
+  subroutine fftini(n_x,n_y,bcxy,c_or_f,arrplan,normfft)
+    integer         , intent(in ), dimension(3)     :: n_x,n_y
+    character(len=1), intent(in ), dimension(0:1,2) :: bcxy
+    character(len=1), intent(in ), dimension(2)     :: c_or_f
+    type(c_ptr)     , intent(out), dimension(2,2)   :: arrplan
+    real(rp)        , intent(out)                   :: normfft
+    integer(c_int) :: kf, kb
+    real(rp) :: norm(2)
+    integer(c_size_t) :: ex, ey
+    ex = int(n_x(1), c_size_t)*n_x(2)*n_x(3)
+    ey = int(n_y(1), c_size_t)*n_y(2)*n_y(3)
+    normfft = 1.0_rp
+    call find_fft(bcxy(:,1), c_or_f(1), kf, kb, norm)
+    arrplan(1,1) = new_plan(kf, n_x(1), 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
+    arrplan(2,1) = new_plan(kb, n_x(1), 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
+    normfft = normfft*norm(1)*(n_x(1) + norm(2))
+    call find_fft(bcxy(:,2), c_or_f(2), kf, kb, norm)
+    arrplan(1,2) = new_plan(kf, n_y(2), n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
+    arrplan(2,2) = new_plan(kb, n_y(2), n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
+    normfft = normfft*norm(1)*(n_y(2) + norm(2))
+    normfft = normfft**(-1)
+  end subroutine fftini
+  !
+  ! fft (src/fft.f90:181-193): executes a plan of fftini in place on arr,
+  ! unnormalised; host arrays are staged through a device buffer.
+  subroutine fft(plan,arr)
+    type(c_ptr), intent(in   )                                       :: plan
+    real(rp)   , intent(inout), dimension(:,:,:), target, contiguous :: arr
+    type(line_plan), pointer :: lp
+    type(c_ptr) :: d
+    integer(c_size_t) :: bytes
+    if (plan_index(plan) == 0) error stop 'flutas_amd fft: plan not made by fftini'
+    call c_f_pointer(plan, lp)
+    if (size(arr, kind=c_size_t) /= lp%elems) error stop 'flutas_amd fft: array size differs from the plan'
+    bytes = 8_c_size_t*lp%elems
+    d = stage_in(6, c_loc(arr), bytes, .true.)
+    call flutas_check(flutas_fft_guru(lp%kind, lp%n, lp%stride, lp%howmany1, lp%dist1, &
+                                      lp%howmany2, lp%dist2, d), 'fft')
+    call stage_out(c_loc(arr), d, bytes)
+  end subroutine fft
+  !
+  ! fftend (src/fft.f90:159-179): frees the plans of fftini, or the solver
+  ! plan initsolver keeps in arrplan(1,1).
   subroutine fftend(arrplan)
     type(c_ptr), intent(inout), dimension(2,2) :: arrplan
-    if (c_associated(arrplan(1,1))) call flutas_check(flutas_psolve_destroy(arrplan(1,1)), 'fftend')
+    type(line_plan), pointer :: lp
+    type(c_ptr), allocatable :: kept(:)
+    integer :: i, j, q
+    if (c_associated(arrplan(1,1)) .and. plan_index(arrplan(1,1)) == 0) then
+      call flutas_check(flutas_psolve_destroy(arrplan(1,1)), 'fftend')
+    else
+      do j = 1, 2
+        do i = 1, 2
+          q = plan_index(arrplan(i,j))
+          if (q == 0) cycle
+          call c_f_pointer(arrplan(i,j), lp)
+          deallocate(lp)
+          kept = [issued(1:q-1), issued(q+1:)]
+          call move_alloc(kept, issued)
+        end do
+      end do
+    end if
     arrplan(:,:) = c_null_ptr
     call stage_release()
   end subroutine fftend

@@ -192,6 +192,15 @@ module flutas_capi
       integer(c_int) :: istep
       integer(c_int) :: rc
     end function
+    ! r2r lines in the geometry of fftini's guru plans (src/fft.f90:75-124)
+    function flutas_fft_guru(kind, n, stride, howmany1, dist1, howmany2, dist2, dat) &
+        bind(C, name="flutas_fft_guru") result(rc)
+      import :: c_int, c_long_long, c_ptr
+      integer(c_int), value :: kind, n, howmany1, howmany2
+      integer(c_long_long), value :: stride, dist1, dist2
+      type(c_ptr), value :: dat
+      integer(c_int) :: rc
+    end function
     function flutas_initgrid(n, gr, lz, nh_d, dzc, dzf, zc, zf) bind(C, name="flutas_initgrid") result(rc)
       import :: c_int, c_double
       integer(c_int), value :: n, nh_d

@@ -131,6 +131,15 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
  * REDFT10(5), REDFT11(6), RODFT01(8), RODFT10(9), RODFT11(10); n even. */
 int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data);
+/* The same transform over a two-level batch, in the geometry of the guru
+ * plans of fftini (src/fft.f90:75-86 for x, :113-124 for y): howmany1
+ * lines at distance dist1, repeated howmany2 (<= 65535) times at distance
+ * dist2; element stride `stride`.  x of an (nx, ny, nz) array: n = nx,
+ * stride 1, (ny, nx), (nz, nx*ny); y: n = ny, stride nx, (nx, 1),
+ * (nz, nx*ny).  Synchronous, like flutas_fft_lines. */
+int flutas_fft_guru(int kind, int n, long long stride, int howmany1,
+                    long long dist1, int howmany2, long long dist2,
+                    double *data);
 
 /* ---- batched solves (no counterpart in the reference, which solves one
  *      problem per call; for many independent small solves) ------------- */

@@ -55,6 +55,18 @@ def r2r(kind, x, direct=False):
     return out
 
 
+def find_fft(bc, c_or_f="c"):
+    """find_fft (src/fft.f90:233-291) of the pair bc ("PP", "NN", ...):
+    (kind_fwd, kind_bwd, norm) with FFTW's kind numbers."""
+    kf, kb = C.c_int(), C.c_int()
+    norm = np.zeros(2)
+    rc = lib().or_find_fft(C.c_char(bc[0].encode()), C.c_char(bc[1].encode()),
+                           C.c_char(c_or_f.encode()), C.byref(kf), C.byref(kb), _p(norm))
+    if rc:
+        raise ValueError(f"find_fft: unknown pair {bc!r} {c_or_f!r}")
+    return kf.value, kb.value, (float(norm[0]), float(norm[1]))
+
+
 def eigenvalues(n, bc, c_or_f="c"):
     lam = np.zeros(n)
     lib().or_eigenvalues(C.c_int(n), C.c_char(bc[0].encode()),

@@ -7,6 +7,8 @@ import re
 import numpy as np
 import pytest
 
+from cases import z_metrics
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "flutas_amd.h")
 
@@ -139,3 +141,37 @@ def test_fortran_mod_load_round_trip(tmp_path):
         np.testing.assert_array_equal(raw[q * npts:(q + 1) * npts].reshape(n, order="F"), ref)
     assert tuple(raw[-2:]) == (1.5, 42.0)
     assert (tmp_path / "scalar.out").read_text() == "  0.1250000E+00  0.1000000E-02 00001234\n"
+
+
+FFT_DEMO = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "fft_demo")
+
+
+@pytest.mark.parametrize("bcs", ["PPNNcc", "NNDDcc", "NDDNcc", "DDPPcc", "PPNDff", "DNPPfc",
+                                 "NDNNfc"])
+def test_fortran_fftini_normfft(oracle, bcs):
+    """The drop-in mod_fft's fftini (src/fft.f90:24-157) gives the oracle's
+    fftini normalisation for nx = 12, ny = 10, face-centred pairs included;
+    it does no device work."""
+    import subprocess
+    if not os.path.exists(FFT_DEMO):
+        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
+    r = subprocess.run([FFT_DEMO, "-n", bcs], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = float(r.stdout.split()[0])
+    nz = 4
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, 0.0, 1.0)
+    s = oracle.Solver((12, 10, nz), (12.0, 10.0, 4.0), dzci, dzfi, bcs[:4] + "NN",
+                      c_or_f=bcs[4:] + "c")
+    assert got == s.normfft, (got, s.normfft)
+
+
+@pytest.mark.parametrize("bcs", ["NNPPfc", "PPDDcf"])
+def test_fortran_fftini_rejects_face_nn_dd(bcs):
+    """Face-centred NN and DD (REDFT00/RODFT00) are not built: fftini stops
+    with a message instead of returning a plan."""
+    import subprocess
+    if not os.path.exists(FFT_DEMO):
+        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
+    r = subprocess.run([FFT_DEMO, "-n", bcs], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "REDFT00/RODFT00" in r.stdout + r.stderr

@@ -101,3 +101,54 @@ def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
         assert relmax(g, o) <= 1e-12
     assert divmax_g == pytest.approx(divmax_o, rel=1e-12)
     assert abs(divtot_g - divtot_o) <= 1e-12 * divmax_o * nx * ny * nz
+
+
+FFT_DEMO = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "fft_demo")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ng,bcs", [
+    ((12, 10, 3), "PPNNcc"),
+    ((32, 24, 4), "NNDDcc"),
+    ((16, 12, 2), "NDDNcc"),
+    ((12, 16, 3), "PPNDff"),
+    ((20, 12, 2), "DNPPfc"),
+])
+def test_fortran_fft_drop_in(oracle, tmp_path, ng, bcs):
+    """The drop-in mod_fft as solver_cpu drives it (src/solver_cpu.f90:59-89)
+    on host arrays: fft(arrplan(1,1)) is the x transform of every line and
+    fft(arrplan(1,2)) the y transform, each equal to the oracle's r2r of the
+    kind find_fft selects; the backward pair times normfft returns the
+    input."""
+    if not os.path.exists(FFT_DEMO):
+        pytest.fail("fft_demo is not built (make -C flutas_amd/fortran)")
+    nx, ny, nz = ng
+    a = np.asfortranarray(np.random.default_rng(sum(ng)).standard_normal(ng))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as fh:
+        fh.write(np.array(ng, np.int32).tobytes())
+        fh.write(bcs.encode())
+        fh.write(a.tobytes(order="F"))
+    print(f"fft_demo {ng} {bcs}", flush=True)
+    r = subprocess.run([FFT_DEMO, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fft_demo ok" in r.stdout
+    raw = np.fromfile(fout, dtype=np.float64)
+    npts = a.size
+    normfft = raw[0]
+    bx, bxy, back = (raw[1 + q * npts:1 + (q + 1) * npts].reshape(ng, order="F") for q in range(3))
+    kx, _, _ = oracle.find_fft(bcs[0:2], bcs[4])
+    ky, _, _ = oracle.find_fft(bcs[2:4], bcs[5])
+    ref_x = np.empty(ng)
+    for j in range(ny):
+        for k in range(nz):
+            ref_x[:, j, k] = oracle.r2r(kx, a[:, j, k])
+    assert relmax(bx, ref_x) <= 1e-13
+    ref_xy = np.empty(ng)
+    for i in range(nx):
+        for k in range(nz):
+            ref_xy[i, :, k] = oracle.r2r(ky, bx[i, :, k])
+    assert relmax(bxy, ref_xy) <= 1e-13
+    assert relmax(back, a) <= 1e-13
+    assert normfft > 0.0

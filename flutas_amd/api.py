@@ -469,11 +469,13 @@ def fft_guru(kind, data, n, stride, howmany1, dist1, howmany2=1, dist2=0):
                                C.c_int(howmany2), C.c_longlong(dist2), _dptr(data)))
 
 
-def initgrid(n, gr, lz, nh_d=1):
-    """src/initgrid.f90:17-97; returns dzc, dzf, zc, zf over (1-nh_d:n+nh_d)."""
+def initgrid(n, gr, lz, nh_d=1, inivel="zer"):
+    """src/initgrid.f90:17-97 (inivel "hcl"/"hcp" cluster at the lower end);
+    returns dzc, dzf, zc, zf over (1-nh_d:n+nh_d)."""
     arrs = [np.zeros(n + 2 * nh_d) for _ in range(4)]
-    _check(lib.flutas_initgrid(n, C.c_double(gr), C.c_double(lz), nh_d,
-                               *[a.ctypes.data_as(_D) for a in arrs]))
+    _check(lib.flutas_initgrid_inivel(C.c_char_p(inivel.encode()), n, C.c_double(gr),
+                                      C.c_double(lz), nh_d,
+                                      *[a.ctypes.data_as(_D) for a in arrs]))
     return arrs
 
 

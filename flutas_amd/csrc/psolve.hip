@@ -708,15 +708,20 @@ int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
   return FLUTAS_OK;
 }
 
-// initgrid (src/initgrid.f90:17-97) with gridpoint_cluster_two_end
-// (:102-118); arrays (1-nh_d : n+nh_d), g(k) == arr[k + nh_d - 1].
-int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
-                    double *dzf, double *zc, double *zf) {
+// initgrid (src/initgrid.f90:17-97); arrays (1-nh_d : n+nh_d), g(k) ==
+// arr[k + nh_d - 1].  one_end selects gridpoint_cluster_one_end
+// (:120-136) instead of gridpoint_cluster_two_end (:102-118), as inivel
+// 'hcl' / 'hcp' do (:32-39).
+static int initgrid_impl(bool one_end, int n, double gr, double lz, int nh_d,
+                         double *dzc, double *dzf, double *zc, double *zf) {
   if (n < 1 || nh_d < 1 || !dzc || !dzf || !zc || !zf) return fail(FLUTAS_EINVAL, "bad initgrid arguments");
   auto G = [nh_d](double *arr, int k) -> double & { return arr[k + nh_d - 1]; };
   for (int k = 1; k <= n; ++k) {
     const double z0 = (k - 0.0) / (1.0 * n);
-    const double z = (gr != 0.0) ? 0.5 * (1.0 + std::tanh((z0 - 0.5) * gr) / std::tanh(gr / 2.0)) : z0;
+    double z = z0;
+    if (gr != 0.0)
+      z = one_end ? 1.0 * (1.0 + std::tanh((z0 - 1.0) * gr) / std::tanh(gr / 1.0))
+                  : 0.5 * (1.0 + std::tanh((z0 - 0.5) * gr) / std::tanh(gr / 2.0));
     G(zf, k) = z * lz;
   }
   G(zf, 0) = 0.0;
@@ -748,6 +753,19 @@ int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
     G(zc, k) = G(zf, k - 1) + G(dzc, k - 1);
   }
   return FLUTAS_OK;
+}
+
+int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
+                    double *dzf, double *zc, double *zf) {
+  return initgrid_impl(false, n, gr, lz, nh_d, dzc, dzf, zc, zf);
+}
+
+int flutas_initgrid_inivel(const char *inivel, int n, double gr, double lz,
+                           int nh_d, double *dzc, double *dzf, double *zc,
+                           double *zf) {
+  if (!inivel) return fail(FLUTAS_EINVAL, "null inivel");
+  const std::string v(inivel, strnlen(inivel, 3));
+  return initgrid_impl(v == "hcl" || v == "hcp", n, gr, lz, nh_d, dzc, dzf, zc, zf);
 }
 
 }  // extern "C"

@@ -201,6 +201,15 @@ module flutas_capi
       type(c_ptr), value :: dat
       integer(c_int) :: rc
     end function
+    function flutas_initgrid_inivel(inivel, n, gr, lz, nh_d, dzc, dzf, zc, zf) &
+        bind(C, name="flutas_initgrid_inivel") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), dimension(*) :: inivel
+      integer(c_int), value :: n, nh_d
+      real(c_double), value :: gr, lz
+      real(c_double), dimension(*) :: dzc, dzf, zc, zf
+      integer(c_int) :: rc
+    end function
     function flutas_initgrid(n, gr, lz, nh_d, dzc, dzf, zc, zf) bind(C, name="flutas_initgrid") result(rc)
       import :: c_int, c_double
       integer(c_int), value :: n, nh_d

@@ -283,6 +283,12 @@ int flutas_add_interior(int nx, int ny, int nz, const double *pold, double *p);
  * :102-118); host arrays (1-nh_d : n+nh_d). */
 int flutas_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
                     double *dzf, double *zc, double *zf);
+/* initgrid with its inivel argument (src/initgrid.f90:17-39): "hcl" and
+ * "hcp" select gridpoint_cluster_one_end (:120-136), every other value
+ * gridpoint_cluster_two_end.  inivel: the first three characters. */
+int flutas_initgrid_inivel(const char *inivel, int n, double gr, double lz,
+                           int nh_d, double *dzc, double *dzf, double *zc,
+                           double *zf);
 
 #ifdef __cplusplus
 }

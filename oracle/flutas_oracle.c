@@ -824,16 +824,28 @@ void or_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
 }
 
 /* src/initgrid.f90:17-97 with gridpoint_cluster_two_end (:102-118). */
+/* gridpoint_cluster_two_end (src/initgrid.f90:102-118) and
+ * gridpoint_cluster_one_end (:120-136). */
+static double gridpoint(int one_end, double alpha, double z0) {
+  if (alpha == 0.0) return z0;
+  if (one_end) return 1.0 * (1.0 + tanh((z0 - 1.0) * alpha) / tanh(alpha / 1.0));
+  return 0.5 * (1.0 + tanh((z0 - 0.5) * alpha) / tanh(alpha / 2.0));
+}
+
 void or_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
                  double *dzf, double *zc, double *zf) {
+  or_initgrid_inivel("zer", n, gr, lz, nh_d, dzc, dzf, zc, zf);
+}
+
+void or_initgrid_inivel(const char *inivel, int n, double gr, double lz,
+                        int nh_d, double *dzc, double *dzf, double *zc,
+                        double *zf) {
+  /* src/initgrid.f90:32-39: 'hcl' and 'hcp' cluster at the lower end */
+  const int one_end = !strncmp(inivel, "hcl", 3) || !strncmp(inivel, "hcp", 3);
 #define G(a, k) ((a)[(k) + (nh_d) - 1])
   for (int k = 1; k <= n; ++k) {
-    double z0 = (k - 0.0) / (1.0 * n), z;
-    if (gr != 0.0)
-      z = 0.5 * (1.0 + tanh((z0 - 0.5) * gr) / tanh(gr / 2.0));
-    else
-      z = z0;
-    G(zf, k) = z * lz;
+    double z0 = (k - 0.0) / (1.0 * n);
+    G(zf, k) = gridpoint(one_end, gr, z0) * lz;
   }
   G(zf, 0) = 0.0;
   for (int k = 1; k <= n; ++k) G(dzf, k) = G(zf, k) - G(zf, k - 1);

@@ -119,6 +119,11 @@ void or_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
  * (1-nh_d : n+nh_d). */
 void or_initgrid(int n, double gr, double lz, int nh_d, double *dzc,
                  double *dzf, double *zc, double *zf);
+/* initgrid with the reference's inivel selection (src/initgrid.f90:32-39):
+ * one-end clustering for "hcl"/"hcp", two-end otherwise. */
+void or_initgrid_inivel(const char *inivel, int n, double gr, double lz,
+                        int nh_d, double *dzc, double *dzf, double *zc,
+                        double *zf);
 /* The operator the spectral solver inverts: out = L p for the interior, with
  * homogeneous ghost values from the BC pattern (periodic wrap, p0 = p1 for
  * N, p0 = -p1 for D, cell-centred) and the z coefficients a, b, c of

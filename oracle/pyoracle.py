@@ -75,10 +75,10 @@ def eigenvalues(n, bc, c_or_f="c"):
     return lam
 
 
-def initgrid(n, gr, lz, nh_d=1):
+def initgrid(n, gr, lz, nh_d=1, inivel="zer"):
     arrs = [np.zeros(n + 2 * nh_d) for _ in range(4)]
-    lib().or_initgrid(C.c_int(n), C.c_double(gr), C.c_double(lz),
-                      C.c_int(nh_d), *[_p(a) for a in arrs])
+    lib().or_initgrid_inivel(C.c_char_p(inivel.encode()), C.c_int(n), C.c_double(gr),
+                             C.c_double(lz), C.c_int(nh_d), *[_p(a) for a in arrs])
     return arrs  # dzc, dzf, zc, zf over (1-nh_d : n+nh_d)
 
 

@@ -46,6 +46,56 @@ def test_initgrid_matches_oracle(oracle, alpha):
             np.testing.assert_array_equal(g, r)
 
 
+@pytest.mark.parametrize("inivel", ["hcl", "hcp", "zer", "poi"])
+def test_initgrid_inivel_matches_oracle(oracle, inivel):
+    """initgrid's inivel selection (src/initgrid.f90:32-39): 'hcl'/'hcp'
+    cluster at the lower wall, other values at both; the library's host
+    code equals the oracle bit for bit."""
+    from flutas_amd import api
+    for alpha in (0.0, 1.0, 2.0):
+        for n, nh in ((16, 1), (64, 3)):
+            got = api.initgrid(n, alpha, 1.3, nh, inivel=inivel)
+            ref = oracle.initgrid(n, alpha, 1.3, nh, inivel=inivel)
+            for g, r in zip(got, ref):
+                np.testing.assert_array_equal(g, r)
+
+
+@pytest.mark.parametrize("alpha", [0.5, 1.0, 2.0])
+def test_oracle_one_end_grid_properties(oracle, alpha):
+    """What the mathematics fixes for gridpoint_cluster_one_end: faces 0 and
+    n at 0 and lz exactly, spacing growing away from the lower wall, and the
+    first-to-last spacing ratio tending to sech^2(alpha) (the derivative
+    ratio of 1 + tanh((z0-1) alpha)/tanh(alpha) at z0 = 0 and 1); the
+    two-end grid instead has equal first and last spacings."""
+    n, nh, lz = 512, 1, 2.0
+    dzc, dzf, zc, zf = oracle.initgrid(n, alpha, lz, nh, inivel="hcl")
+    F = lambda a, k: a[k + nh - 1]  # (1-nh : n+nh) indexing
+    assert F(zf, 0) == 0.0 and abs(F(zf, n) - lz) <= 1e-15 * lz
+    inner = np.array([F(dzf, k) for k in range(1, n + 1)])
+    assert np.all(np.diff(inner) > 0)
+    assert abs(inner[0] / inner[-1] - 1.0 / np.cosh(alpha) ** 2) < 1e-2
+    _, dzf2, _, _ = oracle.initgrid(n, alpha, lz, nh, inivel="zer")
+    assert abs(F(dzf2, 1) - F(dzf2, n)) < 1e-14
+
+
+def test_fortran_mod_initgrid():
+    """The drop-in mod_initgrid, called as the apps call it, returns the
+    library's grid."""
+    import subprocess
+    from flutas_amd import api
+    demo = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "grid_demo")
+    if not os.path.exists(demo):
+        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
+    for inivel, n, gr, nh in (("hcl", 24, 1.5, 2), ("zer", 16, 1.2, 1)):
+        r = subprocess.run([demo, inivel, str(n), str(gr), "1.25", str(nh)],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        rows = [np.array([float(x) for x in line.split()]) for line in r.stdout.splitlines()]
+        ref = api.initgrid(n, gr, 1.25, nh, inivel=inivel)
+        for g, e in zip(rows, ref):
+            np.testing.assert_allclose(g, e, rtol=1e-16, atol=0)
+
+
 def test_invalid_arguments_rejected_without_device():
     """Argument validation happens before any device call."""
     from flutas_amd import api

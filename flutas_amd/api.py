@@ -12,7 +12,7 @@ The functions keep the reference's names, argument order and meaning:
                                                 (src/fillps.f90:16)
     correc(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dzci,dt,rho0,p,u,v,w,rho)
                                                 (src/correc.f90:16)
-    pres_tw_src / pres_sp_src / chkdiv / boundp / updt_rhs_b / initgrid
+    pres_tw_src / pres_sp_src / chkdiv / boundp / bounduvw / updt_rhs_b / initgrid
 
 Field arguments are DeviceArray objects (Fortran-ordered device
 allocations with the halo included).  Failures raise FlutasError carrying
@@ -382,6 +382,35 @@ def boundp(cbc, n, bc, dl, dzc_0, dzc_n, p):
     dlv = (C.c_double * 3)(*[float(v) for v in dl])
     _check(lib.flutas_boundp(_chars(_bc6(cbc), 6), ngc, bcv, dlv,
                              C.c_double(dzc_0), C.c_double(dzc_n), _dptr(p)))
+
+
+def boundp_halo(cbc, n, bc, nh_d, nh_p, dl, dzc, dzf, p):
+    """boundp (src/bound.f90:146-225) on a DeviceArray with halo nh_p;
+    dzc, dzf: host arrays over (1-nh_d : nz+nh_d)."""
+    ngc = (C.c_int * 3)(*[int(v_) for v_ in n])
+    bcv = (C.c_double * 6)(*[float(x) for x in np.asarray(bc, np.float64).ravel()])
+    dlv = (C.c_double * 3)(*[float(x) for x in dl])
+    dzc = np.ascontiguousarray(dzc, np.float64)
+    dzf = np.ascontiguousarray(dzf, np.float64)
+    _check(lib.flutas_boundp_halo(_chars(_bc6(cbc), 6), ngc, bcv, C.c_int(nh_d), C.c_int(nh_p),
+                                  dlv, dzc.ctypes.data_as(_D), dzf.ctypes.data_as(_D), _dptr(p)))
+
+
+def bounduvw(cbc, n, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w):
+    """bounduvw (src/bound.f90:17-144) on halo-nh_u DeviceArrays u, v, w.
+    cbc: 18 characters (or a (2,3,3) array) and bc: 18 values in the
+    reference's (0:1,3,3) memory order; isoutflow: 6 flags (0:1,3);
+    dzc, dzf: host arrays over (1-nh_d : nz+nh_d)."""
+    cbc = cbc if isinstance(cbc, str) else "".join(np.asarray(cbc).ravel(order="F"))
+    ngc = (C.c_int * 3)(*[int(v_) for v_ in n])
+    bcv = (C.c_double * 18)(*[float(x) for x in np.asarray(bc, np.float64).ravel(order="F")])
+    outv = (C.c_int * 6)(*[int(bool(x)) for x in np.asarray(isoutflow).ravel(order="F")])
+    dlv = (C.c_double * 3)(*[float(x) for x in dl])
+    dzc = np.ascontiguousarray(dzc, np.float64)
+    dzf = np.ascontiguousarray(dzf, np.float64)
+    _check(lib.flutas_bounduvw(_chars(cbc, 18), ngc, bcv, C.c_int(nh_d), C.c_int(nh_u), outv,
+                               dlv, dzc.ctypes.data_as(_D), dzf.ctypes.data_as(_D),
+                               _dptr(u), _dptr(v), _dptr(w)))
 
 
 def updt_rhs_b(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p):

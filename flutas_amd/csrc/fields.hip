@@ -269,6 +269,152 @@ __global__ void k_set_bc(int nx, int ny, int nz, char ctype, int side, int dir,
   else p[base + (nd + 1) * s] = factor + sgn * p[base + nd * s];
 }
 
+// ---- bounduvw: set_bc on u, v, w with halo nh, and outflow ---------------
+// (src/bound.f90:17-144, :227-646, :649-773)
+constexpr int kMaxHalo = 8;
+struct FaceField {
+  double *p;
+  double dr[kMaxHalo];  // spacing of layer q (Neumann factor)
+  double value;
+  char ctype;           // 'P', 'D', 'N'; 0: field not on this face
+  int centered;         // cell-centred component (else face-normal)
+};
+struct FaceArgs {
+  FaceField f[3];
+  int n[3];
+  int nh, dir, side;
+};
+
+// One face of set_bc on up to three fields: one lane per tangential
+// position, halos included (set_bc's loop bounds), each lane running the
+// reference's layer loop in order along its own line, so that a later layer
+// overwrites an earlier one exactly where the reference's does (the upper
+// side of the face-normal kinds writes positions nd + q and nd + 1 + q).
+// Reads are interior positions of the line only, so lanes are independent.
+__global__ void k_set_bc_face(FaceArgs a) {
+#pragma clang fp contract(off)
+  const int d = a.dir, t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+  const int nh = a.nh, nd = a.n[d];
+  const long long ext[3] = {a.n[0] + 2LL * nh, a.n[1] + 2LL * nh, a.n[2] + 2LL * nh};
+  const long long st[3] = {1, ext[0], ext[0] * ext[1]};
+  const long long b1 = blockIdx.x * (long long)blockDim.x + threadIdx.x, b2 = blockIdx.y;
+  if (b1 >= ext[t1] || b2 >= ext[t2]) return;
+  const long long base = b1 * st[t1] + b2 * st[t2], s = st[d];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const char t = a.f[c].ctype;
+    if (!t) continue;
+    double *p = a.f[c].p;
+    auto L = [&](int pos) -> double & { return p[base + (long long)(pos - 1 + nh) * s]; };
+    const bool cen = a.f[c].centered != 0;
+    double sgn = 0.0;
+    if (t == 'D' && cen) sgn = -1.0;
+    if (t == 'N') sgn = +1.0;
+    for (int q = 0; q < nh; ++q) {
+      double f = a.f[c].value;
+      if (t == 'D' && cen) f = +2.0 * f;
+      if (t == 'N') f = (a.side == 0) ? -a.f[c].dr[q] * f : a.f[c].dr[q] * f;
+      if (t == 'P') {
+        L(0 - q) = L(nd - q);
+        L(nd + 1 + q) = L(1 + q);
+      } else if (cen) {
+        if (a.side == 0) L(0 - q) = f + sgn * L(1 + q);
+        else L(nd + 1 + q) = f + sgn * L(nd - q);
+      } else if (t == 'D') {
+        if (a.side == 0) {
+          L(0 - q) = f;
+        } else {
+          L(nd + q) = f;
+          L(nd + 1 + q) = L(nd - 1 - q);
+        }
+      } else {
+        if (a.side == 0) {
+          L(0 - q) = 1.0 * f + L(1 + q);
+        } else {
+          L(nd + q) = 1.0 * f + L(nd - 1 - q);
+          L(nd + 1 + q) = 2.0 * f + L(nd - 1 - q);
+        }
+      }
+    }
+  }
+}
+
+// outflow (src/bound.f90:649-773): the velocity normal to an outflow face
+// from zero divergence, layers q = 0..nh on the upper side and 0..nh-1 on
+// the lower side; one lane per tangential interior position.  dzf: device
+// (1-nh_d : nz+nh_d) for the x and y faces; dzq: dzf at the z-face layers
+// (upper: dzf(nz+q); lower: dzf(-q)).
+struct OutflowArgs {
+  double *u, *v, *w;
+  const double *dzf;
+  double dzq[kMaxHalo + 1];
+  double dx, dy, dxi, dyi;
+  int n[3];
+  int nh, nh_d, dir, side;
+};
+__global__ void k_outflow(OutflowArgs a) {
+#pragma clang fp contract(off)
+  const int nx = a.n[0], ny = a.n[1], nz = a.n[2], nh = a.nh;
+  const long long sx = nx + 2LL * nh, sy = sx * (ny + 2LL * nh);
+  auto F = [&](double *f, int i, int j, int k) -> double & {
+    return f[(i - 1 + nh) + sx * (j - 1 + nh) + sy * (k - 1 + nh)];
+  };
+  const int t1 = blockIdx.x * blockDim.x + threadIdx.x + 1, t2 = blockIdx.y + 1;
+  const int qmin = nh - 1;
+  double *u = a.u, *v = a.v, *w = a.w;
+  if (a.dir == 0) {
+    const int j = t1, k = t2;
+    if (j > ny || k > nz) return;
+    const double dzfi = 1.0 / a.dzf[k + a.nh_d - 1];
+    if (a.side == 1) {
+      const int i = nx;
+      for (int q = 0; q <= qmin + 1; ++q)
+        F(u, i + q, j, k) = F(u, i - 1 - q, j, k) -
+            a.dx * ((F(v, i + q, j, k) - F(v, i + q, j - 1, k)) * a.dyi +
+                    (F(w, i + q, j, k) - F(w, i + q, j, k - 1)) * dzfi);
+    } else {
+      const int i = 0;
+      for (int q = 0; q <= qmin; ++q)
+        F(u, i - q, j, k) = F(u, i + 1 + q, j, k) +
+            a.dx * ((F(v, i + 1 + q, j, k) - F(v, i + 1 + q, j - 1, k)) * a.dyi +
+                    (F(w, i + 1 + q, j, k) - F(w, i + 1 + q, j, k - 1)) * dzfi);
+    }
+  } else if (a.dir == 1) {
+    const int i = t1, k = t2;
+    if (i > nx || k > nz) return;
+    const double dzfi = 1.0 / a.dzf[k + a.nh_d - 1];
+    if (a.side == 1) {
+      const int j = ny;
+      for (int q = 0; q <= qmin + 1; ++q)
+        F(v, i, j + q, k) = F(v, i, j - 1 - q, k) -
+            a.dy * ((F(u, i, j + q, k) - F(u, i - 1, j + q, k)) * a.dxi +
+                    (F(w, i, j + q, k) - F(w, i, j + q, k - 1)) * dzfi);
+    } else {
+      const int j = 0;
+      for (int q = 0; q <= qmin; ++q)
+        F(v, i, j - q, k) = F(v, i, j + 1 + q, k) +
+            a.dy * ((F(u, i, j + 1 + q, k) - F(u, i - 1, j + 1 + q, k)) * a.dxi +
+                    (F(w, i, j + 1 + q, k) - F(w, i, j + 1 + q, k - 1)) * dzfi);
+    }
+  } else {
+    const int i = t1, j = t2;
+    if (i > nx || j > ny) return;
+    if (a.side == 1) {
+      const int k = nz;
+      for (int q = 0; q <= qmin + 1; ++q)
+        F(w, i, j, k + q) = F(w, i, j, k - 1 - q) -
+            a.dzq[q] * ((F(u, i, j, k + q) - F(u, i - 1, j, k + q)) * a.dxi +
+                        (F(v, i, j, k + q) - F(v, i, j - 1, k + q)) * a.dyi);
+    } else {
+      const int k = 0;
+      for (int q = 0; q <= qmin; ++q)
+        F(w, i, j, k - q) = F(w, i, j, k + 1 + q) +
+            a.dzq[q] * ((F(u, i, j, k + 1 + q) - F(u, i - 1, j, k + 1 + q)) * a.dxi +
+                        (F(v, i, j, k + 1 + q) - F(v, i, j - 1, k + 1 + q)) * a.dyi);
+    }
+  }
+}
+
 // ---- updt_rhs_b (src/bound.f90:829-944) ------------------------------------
 __global__ void k_add_face(int nx, int ny, int nz, int dir, int q,
                            const double *__restrict__ rhs,
@@ -675,6 +821,174 @@ int flutas_boundp(const char cbc[6], const int n[3], const double bc[6],
   FL_TRY(face(cbc[1], 1, 0, bc[1], dl[0]));
   if (cbc[2] != 'P') { FL_TRY(face(cbc[2], 0, 1, bc[2], dl[1])); FL_TRY(face(cbc[3], 1, 1, bc[3], dl[1])); }
   if (cbc[4] != 'P') { FL_TRY(face(cbc[4], 0, 2, bc[4], dzc_0)); FL_TRY(face(cbc[5], 1, 2, bc[5], dzc_n)); }
+  return FLUTAS_OK;
+}
+
+// bounduvw at one rank, in the decomposition model of flutas_boundp:
+// periodic y and z halos (updthalo's self-exchange) first, then the x faces
+// (u face-normal, v and w centred), the y and z faces that are not periodic,
+// then the outflow faces in the order x, y, z, lower side first.  One launch
+// per face keeps the reference's face order, which decides the halo edges
+// and corners.
+int flutas_bounduvw(const char cbc[18], const int n[3], const double bc[18],
+                    int nh_d, int nh_u, const int isoutflow[6],
+                    const double dl[3], const double *dzc, const double *dzf,
+                    double *u, double *v, double *w) {
+  if (!cbc || !n || !bc || !isoutflow || !dl || !dzc || !dzf || !u || !v || !w)
+    return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
+  if (nh_u < 1 || nh_u > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_u must be in 1..8");
+  if (nh_d < nh_u) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_u (dzc, dzf at the halo layers)");
+  for (int d = 0; d < 3; ++d)
+    if (n[d] + 2 * nh_u > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
+  auto CB = [&](int s, int d, int c) { return cbc[s + 2 * d + 6 * c]; };
+  auto BV = [&](int s, int d, int c) { return bc[s + 2 * d + 6 * c]; };
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) {
+      for (int s = 0; s < 2; ++s)
+        if (CB(s, d, c) != 'P' && CB(s, d, c) != 'D' && CB(s, d, c) != 'N')
+          return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+      if ((CB(0, d, c) == 'P') != (CB(1, d, c) == 'P'))
+        return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
+    }
+  hipStream_t s = current_stream();
+  double *fld[3] = {u, v, w};
+  const int nz = n[2];
+  auto Z = [&](const double *a, int k) { return a[k + nh_d - 1]; };
+  auto launch = [&](FaceArgs &a) -> int {
+    const int t1 = (a.dir == 0) ? 1 : 0, t2 = (a.dir == 2) ? 1 : 2;
+    const int e1 = n[t1] + 2 * nh_u, e2 = n[t2] + 2 * nh_u;
+    hipLaunchKernelGGL(k_set_bc_face, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  auto face = [&](int dir, int side) {
+    FaceArgs a{};
+    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+    a.nh = nh_u;
+    a.dir = dir;
+    a.side = side;
+    return a;
+  };
+  // updthalo in y and z (periodic directions)
+  for (int d = 1; d <= 2; ++d) {
+    FaceArgs a = face(d, 0);
+    for (int c = 0; c < 3; ++c)
+      if (CB(0, d, c) == 'P') {
+        a.f[c].p = fld[c];
+        a.f[c].ctype = 'P';
+        a.f[c].centered = 1;
+      }
+    if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int side = 0; side < 2; ++side) {
+      FaceArgs a = face(d, side);
+      for (int c = 0; c < 3; ++c) {
+        if (d > 0 && CB(side, d, c) == 'P') continue;  // done by updthalo
+        FaceField &F = a.f[c];
+        F.p = fld[c];
+        F.ctype = CB(side, d, c);
+        F.centered = (c != d);
+        F.value = BV(side, d, c);
+        for (int q = 0; q < nh_u; ++q) {
+          if (d < 2) F.dr[q] = dl[d];
+          else if (c < 2) F.dr[q] = (side == 0) ? Z(dzc, -q) : Z(dzc, nz + q);
+          else F.dr[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
+        }
+      }
+      if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
+    }
+  // outflow faces
+  bool need_dzf = false;
+  for (int d = 0; d < 2; ++d) need_dzf = need_dzf || isoutflow[2 * d] || isoutflow[2 * d + 1];
+  double *dzf_d = nullptr;
+  const size_t nzh = (size_t)nz + 2 * nh_d;
+  if (need_dzf) {
+    FL_HIP(hipMalloc((void **)&dzf_d, sizeof(double) * nzh));
+    hipError_t e = hipMemcpy(dzf_d, dzf, sizeof(double) * nzh, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(dzf_d);
+      return fail(FLUTAS_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+  }
+  int rc = FLUTAS_OK;
+  for (int d = 0; d < 3 && rc == FLUTAS_OK; ++d)
+    for (int side = 0; side < 2 && rc == FLUTAS_OK; ++side) {
+      if (!isoutflow[side + 2 * d]) continue;
+      OutflowArgs a{};
+      a.u = u; a.v = v; a.w = w;
+      a.dzf = dzf_d;
+      // layers written: 0..nh_u on the upper side, 0..nh_u-1 on the lower
+      for (int q = 0; q <= (side == 1 ? nh_u : nh_u - 1); ++q)
+        a.dzq[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
+      a.dx = dl[0]; a.dy = dl[1];
+      a.dxi = 1.0 / dl[0]; a.dyi = 1.0 / dl[1];
+      for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+      a.nh = nh_u; a.nh_d = nh_d; a.dir = d; a.side = side;
+      const int t1 = (d == 0) ? n[1] : n[0], t2 = (d == 2) ? n[1] : n[2];
+      hipLaunchKernelGGL(k_outflow, dim3((t1 + 255) / 256, t2), dim3(256), 0, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(FLUTAS_EDEVICE, std::string("k_outflow: ") + hipGetErrorString(e));
+    }
+  if (dzf_d) {
+    hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(dzf_d);
+    if (rc == FLUTAS_OK && e != hipSuccess) rc = fail(FLUTAS_EDEVICE, hipGetErrorString(e));
+  }
+  return rc;
+}
+
+// boundp for a field with halo nh_p (src/bound.f90:146-225): the face
+// launches of flutas_bounduvw on one cell-centred field.
+int flutas_boundp_halo(const char cbc[6], const int n[3], const double bc[6],
+                       int nh_d, int nh_p, const double dl[3],
+                       const double *dzc, const double *dzf, double *p) {
+  (void)dzf;
+  if (!cbc || !n || !bc || !dl || !dzc || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
+  if (nh_p < 1 || nh_p > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_p must be in 1..8");
+  if (nh_d < nh_p) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_p (dzc at the halo layers)");
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] + 2 * nh_p > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
+    for (int s = 0; s < 2; ++s)
+      if (cbc[2 * d + s] != 'P' && cbc[2 * d + s] != 'D' && cbc[2 * d + s] != 'N')
+        return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+    if ((cbc[2 * d] == 'P') != (cbc[2 * d + 1] == 'P'))
+      return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
+  }
+  hipStream_t s = current_stream();
+  const int nz = n[2];
+  auto run = [&](int dir, int side, char t, double value, double dr0, bool zlo, bool zhi) -> int {
+    FaceArgs a{};
+    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+    a.nh = nh_p;
+    a.dir = dir;
+    a.side = side;
+    FaceField &F = a.f[0];
+    F.p = p;
+    F.ctype = t;
+    F.centered = 1;
+    F.value = value;
+    for (int q = 0; q < nh_p; ++q)
+      F.dr[q] = zlo ? dzc[-q + nh_d - 1] : (zhi ? dzc[nz + q + nh_d - 1] : dr0);
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    const int e1 = n[t1] + 2 * nh_p, e2 = n[t2] + 2 * nh_p;
+    hipLaunchKernelGGL(k_set_bc_face, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  // updthalo in y and z (periodic), then x, y, z faces as set_bc
+  if (cbc[2] == 'P') FL_TRY(run(1, 0, 'P', 0.0, 0.0, false, false));
+  if (cbc[4] == 'P') FL_TRY(run(2, 0, 'P', 0.0, 0.0, false, false));
+  FL_TRY(run(0, 0, cbc[0], bc[0], dl[0], false, false));
+  FL_TRY(run(0, 1, cbc[1], bc[1], dl[0], false, false));
+  if (cbc[2] != 'P') FL_TRY(run(1, 0, cbc[2], bc[2], dl[1], false, false));
+  if (cbc[3] != 'P') FL_TRY(run(1, 1, cbc[3], bc[3], dl[1], false, false));
+  if (cbc[4] != 'P') FL_TRY(run(2, 0, cbc[4], bc[4], 0.0, true, false));
+  if (cbc[5] != 'P') FL_TRY(run(2, 1, cbc[5], bc[5], 0.0, false, true));
   return FLUTAS_OK;
 }
 

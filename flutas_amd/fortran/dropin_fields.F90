@@ -183,8 +183,9 @@ contains
   end subroutine chkdiv
 end module mod_chkdiv
 !
-! src/bound.f90:146-225 (boundp) and :829-944 (updt_rhs_b), one rank,
-! pressure halo nh_p = 1.
+! src/bound.f90:17-144 (bounduvw, velocity halo nh_u <= 8), :146-225
+! (boundp, any field with halo nh_p <= 8) and :829-944 (updt_rhs_b,
+! pressure halo 1), one rank.
 module mod_bound
   use, intrinsic :: iso_c_binding
   use flutas_capi
@@ -192,24 +193,65 @@ module mod_bound
   use flutas_field_sizes
   implicit none
   private
-  public :: boundp, updt_rhs_b
+  public :: bounduvw, boundp, updt_rhs_b
   integer, parameter :: rp = c_double
 contains
-  ! value of a metric array element that may live in device memory
-  function metric_at(arr, idx) result(val)
+  ! host copy of a metric array that may live in device memory
+  subroutine metric_to_host(arr, out)
     real(rp), intent(in), dimension(:), target, contiguous :: arr
-    integer, intent(in) :: idx
-    real(rp) :: val
-    real(rp), target :: tmp
+    real(rp), intent(out), dimension(:), target, contiguous :: out
     integer(c_int) :: isdev
     call flutas_check(flutas_pointer_is_device(c_loc(arr), isdev), 'pointer query')
     if (isdev == 1) then
-      call flutas_check(flutas_memcpy_d2h(c_loc(tmp), c_loc(arr(idx)), 8_c_size_t), 'metric copy')
-      val = tmp
+      call flutas_check(flutas_memcpy_d2h(c_loc(out), c_loc(arr), nbytes(size(arr))), 'metric copy')
     else
-      val = arr(idx)
+      out = arr
     end if
-  end function metric_at
+  end subroutine metric_to_host
+  !
+  subroutine bounduvw(cbc,n,bc,nh_d,nh_u,halo,isoutflow,dl,dzc,dzf,u,v,w)
+    character(len=1), intent(in   ), dimension(0:1,3,3)                 :: cbc
+    integer         , intent(in   ), dimension(3)                       :: n
+    real(rp)        , intent(in   ), dimension(0:1,3,3)                 :: bc
+    integer         , intent(in   )                                     :: nh_d,nh_u
+    integer         , intent(in   ), dimension(3)                       :: halo
+    logical         , intent(in   ), dimension(0:1,3)                   :: isoutflow
+    real(rp)        , intent(in   ), dimension(3)                       :: dl
+    real(rp)        , intent(in   ), dimension(1-nh_d:), target, contiguous :: dzc,dzf
+    real(rp)        , intent(inout), dimension(1-nh_u:,1-nh_u:,1-nh_u:), target, contiguous :: u,v,w
+    character(kind=c_char) :: cbc_c(18)
+    real(c_double) :: bc_c(18)
+    integer(c_int) :: nn(3), out_c(6)
+    real(rp), allocatable, target :: hdzc(:), hdzf(:)
+    type(c_ptr) :: du, dv, dw
+    integer :: q, s, d, c
+    do c = 1, 3
+      do d = 1, 3
+        do s = 0, 1
+          q = s + 2*(d-1) + 6*(c-1) + 1
+          cbc_c(q) = cbc(s,d,c)
+          bc_c(q) = bc(s,d,c)
+        end do
+      end do
+    end do
+    do d = 1, 3
+      do s = 0, 1
+        out_c(s + 2*(d-1) + 1) = merge(1_c_int, 0_c_int, isoutflow(s,d))
+      end do
+    end do
+    nn = int(n, c_int)
+    allocate(hdzc(size(dzc)), hdzf(size(dzf)))
+    call metric_to_host(dzc, hdzc)
+    call metric_to_host(dzf, hdzf)
+    du = stage_in(1, c_loc(u), nbytes(size(u)), .true.)
+    dv = stage_in(2, c_loc(v), nbytes(size(v)), .true.)
+    dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
+    call flutas_check(flutas_bounduvw(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_u, c_int), out_c, &
+                                      dl, hdzc, hdzf, du, dv, dw), 'bounduvw')
+    call stage_out(c_loc(u), du, nbytes(size(u)))
+    call stage_out(c_loc(v), dv, nbytes(size(v)))
+    call stage_out(c_loc(w), dw, nbytes(size(w)))
+  end subroutine bounduvw
   !
   subroutine boundp(cbc,n,bc,nh_d,nh_p,halo,dl,dzc,dzf,p)
     character(len=1), intent(in   ), dimension(0:1,3)                   :: cbc
@@ -221,11 +263,11 @@ contains
     real(rp)        , intent(in   ), dimension(1-nh_d:), target, contiguous :: dzc,dzf
     real(rp)        , intent(inout), dimension(1-nh_p:,1-nh_p:,1-nh_p:), target, contiguous :: p
     character(kind=c_char) :: cbc_c(6)
-    real(c_double) :: bc_c(6), dzc0, dzcn
+    real(c_double) :: bc_c(6)
     integer(c_int) :: nn(3)
+    real(rp), allocatable, target :: hdzc(:), hdzf(:)
     type(c_ptr) :: dp
     integer :: q, s, d
-    if (nh_p /= 1) error stop 'flutas_amd boundp: pressure halo nh_p = 1 only'
     do d = 1, 3
       do s = 0, 1
         q = 2*(d-1) + s + 1
@@ -234,12 +276,12 @@ contains
       end do
     end do
     nn = int(n, c_int)
-    ! dzc(0) and dzc(n(3)) sit at positions nh_d and n(3)+nh_d of the
-    ! 1-based view of dzc(1-nh_d:)
-    dzc0 = metric_at(dzc, nh_d)
-    dzcn = metric_at(dzc, n(3) + nh_d)
+    allocate(hdzc(size(dzc)), hdzf(size(dzf)))
+    call metric_to_host(dzc, hdzc)
+    call metric_to_host(dzf, hdzf)
     dp = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
-    call flutas_check(flutas_boundp(cbc_c, nn, bc_c, dl, dzc0, dzcn, dp), 'boundp')
+    call flutas_check(flutas_boundp_halo(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_p, c_int), dl, &
+                                         hdzc, hdzf, dp), 'boundp')
     call stage_out(c_loc(p), dp, nbytes(size(p)))
   end subroutine boundp
   !

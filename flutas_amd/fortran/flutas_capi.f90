@@ -126,6 +126,27 @@ module flutas_capi
       type(c_ptr), value :: p
       integer(c_int) :: rc
     end function
+    function flutas_boundp_halo(cbc, n, bc, nh_d, nh_p, dl, dzc, dzf, p) &
+        bind(C, name="flutas_boundp_halo") result(rc)
+      import :: c_int, c_double, c_char, c_ptr
+      character(kind=c_char), dimension(*) :: cbc
+      integer(c_int), dimension(3) :: n
+      real(c_double), dimension(*) :: bc, dl, dzc, dzf
+      integer(c_int), value :: nh_d, nh_p
+      type(c_ptr), value :: p
+      integer(c_int) :: rc
+    end function
+    function flutas_bounduvw(cbc, n, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w) &
+        bind(C, name="flutas_bounduvw") result(rc)
+      import :: c_int, c_double, c_char, c_ptr
+      character(kind=c_char), dimension(*) :: cbc
+      integer(c_int), dimension(3) :: n
+      real(c_double), dimension(*) :: bc, dl, dzc, dzf
+      integer(c_int), value :: nh_d, nh_u
+      integer(c_int), dimension(6) :: isoutflow
+      type(c_ptr), value :: u, v, w
+      integer(c_int) :: rc
+    end function
     function flutas_correc(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, p, u, v, w, rho, &
         const_coeff) bind(C, name="flutas_correc") result(rc)
       import :: c_int, c_double, c_ptr

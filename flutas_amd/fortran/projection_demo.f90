@@ -1,8 +1,11 @@
 ! One pressure-projection step through the drop-in modules, in the order of
 ! the reference's single-phase time loop (main__single_phase.f90:705-727):
-!   fillps -> updt_rhs_b -> solver_gpu -> boundp -> correc
+!   fillps -> updt_rhs_b -> solver_gpu -> boundp -> correc -> bounduvw
 ! plus chkdiv of the predicted velocity, with the grid of initgrid and the
-! plan of initsolver.
+! plan of initsolver.  nh_d = max(nh_u, 1) as the reference sets it
+! (main__single_phase.f90:261); the velocity BCs follow the pressure
+! pattern: periodic stays periodic, a pressure N wall is a no-slip (D, 0)
+! wall, a pressure D face a zero-gradient (N) face, no outflow faces.
 !
 !   projection_demo INPUT OUTPUT MODE
 !
@@ -10,8 +13,8 @@
 ! 6 characters cbc (the (0:1,3) order); then u, v, w as
 ! (1-nh_u:n+nh_u)^3 arrays in Fortran order.
 ! OUTPUT (stream binary): f (right-hand side after updt_rhs_b), p (after
-! the solve and boundp), u, v, w (after correc), divtot and divmax of the
-! predicted velocity.
+! the solve and boundp), u, v, w (after correc and bounduvw), divtot and
+! divmax of the predicted velocity.
 ! MODE: "host" passes plain Fortran arrays (the modules stage them through
 ! device buffers); "device" passes HIP device allocations.
 program projection_demo
@@ -23,15 +26,16 @@ program projection_demo
   use mod_fillps,     only: fillps
   use mod_correc,     only: correc
   use mod_chkdiv,     only: chkdiv
-  use mod_bound,      only: boundp, updt_rhs_b
+  use mod_bound,      only: bounduvw, boundp, updt_rhs_b
   implicit none
-  integer, parameter :: rp = c_double, nh_d = 1
+  integer, parameter :: rp = c_double
   character(len=512) :: fin, fout, mode
   integer(c_int32_t) :: hdr(4)
   real(rp) :: par(6)
-  character(len=1) :: cbc(0:1,3), cbc6(6)
-  real(rp) :: bc(0:1,3)
-  integer :: n(3), nh_u, i, u_in, u_out
+  character(len=1) :: cbc(0:1,3), cbc6(6), cbcvel(0:1,3,3)
+  real(rp) :: bc(0:1,3), bcvel(0:1,3,3)
+  logical :: no_outflow(0:1,3)
+  integer :: n(3), nh_u, nh_d, i, u_in, u_out
   real(rp), allocatable, target :: u(:,:,:), v(:,:,:), w(:,:,:), p(:,:,:), rho(:,:,:), f(:,:,:)
   real(rp), pointer, contiguous :: ud(:,:,:), vd(:,:,:), wd(:,:,:), pd(:,:,:)
   type(c_ptr) :: cu, cv, cw, cp
@@ -44,10 +48,18 @@ program projection_demo
   read(u_in) hdr, par, cbc6
   n = hdr(1:3)
   nh_u = hdr(4)
+  nh_d = max(nh_u, 1)
   do i = 1, 6
     cbc(mod(i-1,2), (i-1)/2 + 1) = cbc6(i)
   end do
   bc = 0._rp
+  do i = 1, 3
+    where (cbc == 'P') cbcvel(:,:,i) = 'P'
+    where (cbc == 'N') cbcvel(:,:,i) = 'D'
+    where (cbc == 'D') cbcvel(:,:,i) = 'N'
+  end do
+  bcvel = 0._rp
+  no_outflow = .false.
   allocate(u(1-nh_u:n(1)+nh_u,1-nh_u:n(2)+nh_u,1-nh_u:n(3)+nh_u))
   allocate(v, w, mold=u)
   allocate(p(0:n(1)+1,0:n(2)+1,0:n(3)+1))
@@ -120,6 +132,7 @@ contains
     call solver_gpu(n, [1,1,1], arrplan, normfft, lambdaxy, a, b, c, cbc, ['c','c','c'], pp)
     call boundp(cbc, n, bc, nh_d, 1, [1,1,1], dl, dzc, dzf, pp)
     call correc(n(1), n(2), n(3), nh_d, nh_u, dli(1), dli(2), dli(3), dzci, dt, rho0, pp, uu, vv, ww, rho)
+    call bounduvw(cbcvel, n, bcvel, nh_d, nh_u, [1,1,1], no_outflow, dl, dzc, dzf, uu, vv, ww)
     call fftend(arrplan)
     open(newunit=u_div, file=trim(fout)//'.div', status='replace')
     write(u_div, '(2es25.16)') divtot, divmax

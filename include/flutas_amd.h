@@ -222,6 +222,29 @@ int flutas_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi,
                        const double *dzci, double rho0i, const double *pold,
                        double *u, double *v, double *w);
 
+/* boundp for a field with any halo nh_p (1..8), as the reference calls it
+ * for temperature and VOF fields (src/bound.f90:146-225): p is the device
+ * array (1-nh_p : n+nh_p)^3; dzc, dzf: HOST arrays (1-nh_d : nz+nh_d),
+ * nh_d >= nh_p; the z faces use dzc(-q) and dzc(nz+q) for layer q. */
+int flutas_boundp_halo(const char cbc[6], const int n[3], const double bc[6],
+                       int nh_d, int nh_p, const double dl[3],
+                       const double *dzc, const double *dzf, double *p);
+
+/* bounduvw (src/bound.f90:17-144; set_bc :227-646; outflow :649-773) at
+ * one rank: the halos of the device fields u, v, w (halo nh_u, 1..8).
+ * cbc, bc: the reference's cbc(0:1,3,3), bc(0:1,3,3) in memory order,
+ * [side + 2*dir + 6*comp]; isoutflow: the (0:1,3) flags [side + 2*dir];
+ * dl = (dx, dy, dz); dzc, dzf: HOST arrays (1-nh_d : nz+nh_d), nh_d >=
+ * nh_u.  Periodic y and z halos come first (updthalo's self-exchange),
+ * then the x, y, z faces in the reference's order (u face-normal in x, v in
+ * y, w in z, the others cell-centred), then the outflow faces x, y, z.
+ * Asynchronous unless an x or y outflow face is set (then it returns after
+ * the work is done). */
+int flutas_bounduvw(const char cbc[18], const int n[3], const double bc[18],
+                    int nh_d, int nh_u, const int isoutflow[6],
+                    const double dl[3], const double *dzc, const double *dzf,
+                    double *u, double *v, double *w);
+
 /* updt_rhs_b (src/bound.f90:829-944) at one rank; rhsb* device arrays. */
 int flutas_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
                       const char cbc[6], const double *rhsbx,

@@ -799,6 +799,185 @@ void or_boundp(const char cbc[6], const int n[3], const double bc[6], int nh_d,
   }
 }
 
+/* set_bc (src/bound.f90:227-646) on a field with halo nh: the reference's
+ * cases literally, the layer loop q outermost, so that a later layer
+ * overwrites an earlier one where the reference's does (the upper face of
+ * the face-normal kinds).  dr[q], q < nh: the spacings of the N factor. */
+static void set_bc_nh(const int n[3], int nh, char ctype, int ibound, int idir,
+                      int centered, double rvalue, const double *dr, double *p) {
+  long ext[3] = {n[0] + 2 * nh, n[1] + 2 * nh, n[2] + 2 * nh};
+  long st[3] = {1, ext[0], ext[0] * ext[1]};
+  int d = idir, t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2, nd = n[d];
+  double factor[16], sgn = 0.0;
+  for (int q = 0; q < nh; ++q) factor[q] = rvalue;
+  if (ctype == 'D' && centered) {
+    for (int q = 0; q < nh; ++q) factor[q] = +2.0 * factor[q];
+    sgn = -1.0;
+  }
+  if (ctype == 'N') {
+    for (int q = 0; q < nh; ++q) factor[q] = (ibound == 0) ? -dr[q] * factor[q] : dr[q] * factor[q];
+    sgn = +1.0;
+  }
+#define L(a) p[base + ((long)(a) - 1 + nh) * st[d]]
+  for (int q = 0; q < nh; ++q)
+    for (long b2 = 0; b2 < ext[t2]; ++b2)
+      for (long b1 = 0; b1 < ext[t1]; ++b1) {
+        long base = b1 * st[t1] + b2 * st[t2];
+        const double f = factor[q];
+        if (ctype == 'P') {
+          L(0 - q) = L(nd - q);
+          L(nd + 1 + q) = L(1 + q);
+        } else if (centered) {
+          if (ibound == 0) L(0 - q) = f + sgn * L(1 + q);
+          else L(nd + 1 + q) = f + sgn * L(nd - q);
+        } else if (ctype == 'D') {
+          if (ibound == 0) L(0 - q) = f;
+          else {
+            L(nd + q) = f;
+            L(nd + 1 + q) = L(nd - 1 - q);
+          }
+        } else {
+          if (ibound == 0) L(0 - q) = 1.0 * f + L(1 + q);
+          else {
+            L(nd + q) = 1.0 * f + L(nd - 1 - q);
+            L(nd + 1 + q) = 2.0 * f + L(nd - 1 - q);
+          }
+        }
+      }
+#undef L
+}
+
+/* outflow (src/bound.f90:649-773) at one rank: the velocity normal to the
+ * face from zero divergence; side 1 writes q = 0..nh (nx + q), side 0
+ * q = 0..nh-1 (0 - q). */
+static void outflow_nh(const int n[3], int nh, int nh_d, int dir, int side,
+                       const double dl[3], const double *dzf, double *u,
+                       double *v, double *w) {
+  int nx = n[0], ny = n[1], nz = n[2], qmin = nh - 1;
+  long sx = nx + 2 * nh, sy = sx * (ny + 2 * nh);
+#define F(a, i, j, k) (a)[((long)(i) - 1 + nh) + sx * ((long)(j) - 1 + nh) + sy * ((long)(k) - 1 + nh)]
+#define DZF(k) dzf[(k) + nh_d - 1]
+  double dx = dl[0], dy = dl[1], dxi = 1.0 / dx, dyi = 1.0 / dy;
+  if (dir == 0 && side == 1) {
+    int i = nx;
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int q = 0; q <= qmin + 1; ++q)
+          F(u, i + q, j, k) = F(u, i - 1 - q, j, k) -
+              dx * ((F(v, i + q, j, k) - F(v, i + q, j - 1, k)) * dyi +
+                    (F(w, i + q, j, k) - F(w, i + q, j, k - 1)) * (1.0 / DZF(k)));
+  } else if (dir == 1 && side == 1) {
+    int j = ny;
+    for (int k = 1; k <= nz; ++k)
+      for (int q = 0; q <= qmin + 1; ++q)
+        for (int i = 1; i <= nx; ++i)
+          F(v, i, j + q, k) = F(v, i, j - 1 - q, k) -
+              dy * ((F(u, i, j + q, k) - F(u, i - 1, j + q, k)) * dxi +
+                    (F(w, i, j + q, k) - F(w, i, j + q, k - 1)) * (1.0 / DZF(k)));
+  } else if (dir == 2 && side == 1) {
+    int k = nz;
+    for (int q = 0; q <= qmin + 1; ++q)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+          F(w, i, j, k + q) = F(w, i, j, k - 1 - q) -
+              DZF(k + q) * ((F(u, i, j, k + q) - F(u, i - 1, j, k + q)) * dxi +
+                            (F(v, i, j, k + q) - F(v, i, j - 1, k + q)) * dyi);
+  } else if (dir == 0 && side == 0) {
+    int i = 0;
+    for (int k = 1; k <= nz; ++k)
+      for (int j = 1; j <= ny; ++j)
+        for (int q = 0; q <= qmin; ++q)
+          F(u, i - q, j, k) = F(u, i + 1 + q, j, k) +
+              dx * ((F(v, i + 1 + q, j, k) - F(v, i + 1 + q, j - 1, k)) * dyi +
+                    (F(w, i + 1 + q, j, k) - F(w, i + 1 + q, j, k - 1)) * (1.0 / DZF(k)));
+  } else if (dir == 1 && side == 0) {
+    int j = 0;
+    for (int k = 1; k <= nz; ++k)
+      for (int q = 0; q <= qmin; ++q)
+        for (int i = 1; i <= nx; ++i)
+          F(v, i, j - q, k) = F(v, i, j + 1 + q, k) +
+              dy * ((F(u, i, j + 1 + q, k) - F(u, i - 1, j + 1 + q, k)) * dxi +
+                    (F(w, i, j + 1 + q, k) - F(w, i, j + 1 + q, k - 1)) * (1.0 / DZF(k)));
+  } else {
+    int k = 0;
+    for (int q = 0; q <= qmin; ++q)
+      for (int j = 1; j <= ny; ++j)
+        for (int i = 1; i <= nx; ++i)
+          F(w, i, j, k - q) = F(w, i, j, k + 1 + q) +
+              DZF(k - q) * ((F(u, i, j, k + 1 + q) - F(u, i - 1, j, k + 1 + q)) * dxi +
+                            (F(v, i, j, k + 1 + q) - F(v, i, j - 1, k + 1 + q)) * dyi);
+  }
+#undef F
+#undef DZF
+}
+
+/* bounduvw at one rank (src/bound.f90:17-144), in the same decomposition
+ * model as or_boundp: periodic y and z halos of u, v, w from updthalo (a
+ * self-exchange at one rank), then set_bc for the x faces (u face-normal,
+ * v and w centred), the y faces and the z faces where they are not
+ * periodic, then outflow in the order x, y, z, lower side first.
+ * cbc/bc: the reference's (0:1,3,3) arrays, [side + 2*dir + 6*comp]. */
+void or_bounduvw(const char cbc[18], const int n[3], const double bc[18],
+                 int nh_d, int nh_u, const int isoutflow[6], const double dl[3],
+                 const double *dzc, const double *dzf, double *u, double *v,
+                 double *w) {
+  double *fld[3] = {u, v, w};
+  double dr[16] = {0.0};
+  int nz = n[2];
+#define CB(s, d, c) cbc[(s) + 2 * (d) + 6 * (c)]
+#define BV(s, d, c) bc[(s) + 2 * (d) + 6 * (c)]
+  for (int c = 0; c < 3; ++c) {
+    if (CB(0, 1, c) == 'P') set_bc_nh(n, nh_u, 'P', 0, 1, 1, 0.0, dr, fld[c]);
+    if (CB(0, 2, c) == 'P') set_bc_nh(n, nh_u, 'P', 0, 2, 1, 0.0, dr, fld[c]);
+  }
+  for (int s = 0; s < 2; ++s) {
+    for (int q = 0; q < nh_u; ++q) dr[q] = dl[0];
+    for (int c = 0; c < 3; ++c)
+      set_bc_nh(n, nh_u, CB(s, 0, c), s, 0, c != 0, BV(s, 0, c), dr, fld[c]);
+  }
+  for (int s = 0; s < 2; ++s) {
+    for (int q = 0; q < nh_u; ++q) dr[q] = dl[1];
+    for (int c = 0; c < 3; ++c)
+      if (CB(s, 1, c) != 'P') set_bc_nh(n, nh_u, CB(s, 1, c), s, 1, c != 1, BV(s, 1, c), dr, fld[c]);
+  }
+  for (int s = 0; s < 2; ++s) {
+    for (int q = 0; q < nh_u; ++q) dr[q] = (s == 0) ? dzc[-q + nh_d - 1] : dzc[nz + q + nh_d - 1];
+    for (int c = 0; c < 2; ++c)
+      if (CB(s, 2, c) != 'P') set_bc_nh(n, nh_u, CB(s, 2, c), s, 2, 1, BV(s, 2, c), dr, fld[c]);
+    for (int q = 0; q < nh_u; ++q) dr[q] = (s == 0) ? dzf[-q + nh_d - 1] : dzf[nz + q + nh_d - 1];
+    if (CB(s, 2, 2) != 'P') set_bc_nh(n, nh_u, CB(s, 2, 2), s, 2, 0, BV(s, 2, 2), dr, fld[2]);
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s)
+      if (isoutflow[s + 2 * d]) outflow_nh(n, nh_u, nh_d, d, s, dl, dzf, u, v, w);
+#undef CB
+#undef BV
+}
+
+/* boundp at one rank for a field with halo nh_p (src/bound.f90:146-225):
+ * periodic y and z halos from updthalo, then the centred set_bc of the x
+ * faces (periodic included), and of the y and z faces that are not
+ * periodic; z spacings dzc(-q) and dzc(nz+q). */
+void or_boundp_nh(const char cbc[6], const int n[3], const double bc[6],
+                  int nh_d, int nh_p, const double dl[3], const double *dzc,
+                  const double *dzf, double *p) {
+  (void)dzf;
+  double dr[16] = {0.0};
+  int nz = n[2];
+  if (cbc[2] == 'P') set_bc_nh(n, nh_p, 'P', 0, 1, 1, 0.0, dr, p);
+  if (cbc[4] == 'P') set_bc_nh(n, nh_p, 'P', 0, 2, 1, 0.0, dr, p);
+  for (int q = 0; q < nh_p; ++q) dr[q] = dl[0];
+  set_bc_nh(n, nh_p, cbc[0], 0, 0, 1, bc[0], dr, p);
+  set_bc_nh(n, nh_p, cbc[1], 1, 0, 1, bc[1], dr, p);
+  for (int q = 0; q < nh_p; ++q) dr[q] = dl[1];
+  if (cbc[2] != 'P') set_bc_nh(n, nh_p, cbc[2], 0, 1, 1, bc[2], dr, p);
+  if (cbc[3] != 'P') set_bc_nh(n, nh_p, cbc[3], 1, 1, 1, bc[3], dr, p);
+  for (int q = 0; q < nh_p; ++q) dr[q] = dzc[-q + nh_d - 1];
+  if (cbc[4] != 'P') set_bc_nh(n, nh_p, cbc[4], 0, 2, 1, bc[4], dr, p);
+  for (int q = 0; q < nh_p; ++q) dr[q] = dzc[nz + q + nh_d - 1];
+  if (cbc[5] != 'P') set_bc_nh(n, nh_p, cbc[5], 1, 2, 1, bc[5], dr, p);
+}
+
 /* src/bound.f90:829-944 at one rank. */
 void or_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
                    const char cbc[6], const double *rhsbx, const double *rhsby,

@@ -108,6 +108,19 @@ void or_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
                int nh_d, int nh_u, const double *dzfi, const double *u,
                const double *v, const double *w, double *divtot,
                double *divmax);
+/* boundp for a field with halo nh_p (src/bound.f90:146-225); p is
+ * (1-nh_p : n+nh_p)^3; dzc over (1-nh_d : nz+nh_d), nh_d >= nh_p. */
+void or_boundp_nh(const char cbc[6], const int n[3], const double bc[6],
+                  int nh_d, int nh_p, const double dl[3], const double *dzc,
+                  const double *dzf, double *p);
+/* bounduvw at one rank (src/bound.f90:17-144, set_bc :227-646, outflow
+ * :649-773): u, v, w with halo nh_u (<= 16), cbc/bc the (0:1,3,3) arrays
+ * in memory order [side + 2*dir + 6*comp], isoutflow (0:1,3) as ints,
+ * dzc/dzf over (1-nh_d : nz+nh_d), nh_d >= nh_u. */
+void or_bounduvw(const char cbc[18], const int n[3], const double bc[18],
+                 int nh_d, int nh_u, const int isoutflow[6], const double dl[3],
+                 const double *dzc, const double *dzf, double *u, double *v,
+                 double *w);
 /* boundp at one rank with nh_p = 1 (src/bound.f90:146-225). */
 void or_boundp(const char cbc[6], const int n[3], const double bc[6], int nh_d,
                const double dl[3], const double *dzc, const double *dzf,

@@ -218,3 +218,34 @@ def boundp(cbc, ng, bc, nh_d, dl, dzc, dzf, p):
                     _p(dlv), _p(np.ascontiguousarray(dzc)),
                     _p(np.ascontiguousarray(dzf)), _p(p))
     return p
+
+
+def bounduvw(cbc, ng, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w):
+    """bounduvw at one rank (src/bound.f90:17-144) in place on the halo-nh_u
+    Fortran arrays u, v, w.  cbc: 18 characters (or a (2,3,3) array) and bc:
+    18 values in the (0:1,3,3) memory order [side + 2*dir + 6*comp];
+    isoutflow: 6 flags [side + 2*dir]."""
+    cbc = cbc if isinstance(cbc, str) else "".join(np.asarray(cbc).ravel(order="F"))
+    assert len(cbc) == 18
+    ngc = (C.c_int * 3)(*ng)
+    bcv = np.ascontiguousarray(np.asarray(bc, np.float64).ravel(order="F"))
+    outv = (C.c_int * 6)(*[int(bool(x)) for x in np.asarray(isoutflow).ravel(order="F")])
+    dlv = np.asarray(dl, np.float64)
+    for a in (u, v, w):
+        assert a.flags.f_contiguous and a.dtype == np.float64
+    lib().or_bounduvw(C.c_char_p(cbc.encode()), ngc, _p(bcv), C.c_int(nh_d), C.c_int(nh_u),
+                      outv, _p(dlv), _p(np.ascontiguousarray(dzc)),
+                      _p(np.ascontiguousarray(dzf)), _p(u), _p(v), _p(w))
+    return u, v, w
+
+
+def boundp_nh(cbc, ng, bc, nh_d, nh_p, dl, dzc, dzf, p):
+    """boundp (src/bound.f90:146-225) on a field with halo nh_p, in place."""
+    ngc = (C.c_int * 3)(*ng)
+    bcv = np.asarray(bc, np.float64)
+    dlv = np.asarray(dl, np.float64)
+    assert p.flags.f_contiguous and p.shape == tuple(n + 2 * nh_p for n in ng)
+    lib().or_boundp_nh(C.c_char_p(cbc.encode()), ngc, _p(bcv), C.c_int(nh_d), C.c_int(nh_p),
+                       _p(dlv), _p(np.ascontiguousarray(dzc)), _p(np.ascontiguousarray(dzf)),
+                       _p(p))
+    return p

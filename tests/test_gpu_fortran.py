@@ -1,6 +1,6 @@
 """The Fortran drop-in modules (flutas_amd/fortran) on the GPU: one
 projection step (initsolver, fillps, chkdiv, updt_rhs_b, solver_gpu, boundp,
-correc, fftend) run by the amdflang-built projection_demo, checked against
+correc, bounduvw, fftend) run by the amdflang-built projection_demo, checked against
 the same step through the FP64 oracle.  "host" passes plain Fortran arrays
 (staged through device buffers by the modules), "device" passes HIP device
 allocations."""
@@ -78,21 +78,26 @@ def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
     u_g, v_g, w_g = (x.reshape(hu, order="F") for x in parts[2:])
     divtot_g, divmax_g = np.loadtxt(str(fout) + ".div")
 
-    # the same step through the oracle
-    dzc, dzf, zc, zf = oracle.initgrid(nz, alpha, lengths[2], 1)
+    # the same step through the oracle, with nh_d = max(nh_u, 1) and the
+    # velocity BCs the demo derives from the pressure pattern
+    nh_d = max(nh_u, 1)
+    vel = {"P": "P", "N": "D", "D": "N"}
+    cbcvel = "".join(vel[c] for _ in range(3) for c in cbc)
+    dzc, dzf, zc, zf = oracle.initgrid(nz, alpha, lengths[2], nh_d)
     dzci, dzfi = 1.0 / dzc, 1.0 / dzf
     dli = tuple(n / l for n, l in zip(ng, lengths))
     dl = tuple(1.0 / d for d in dli)
-    s = oracle.Solver(ng, dli, dzci, dzfi, cbc)
+    s = oracle.Solver(ng, dli, dzci, dzfi, cbc, nh_d=nh_d)
     p = np.zeros(hp, order="F")
     uo, vo, wo = (a.copy(order="F") for a in (u, v, w))
-    oracle.fillps(ng, 1, nh_u, dli, dzfi, 1.0 / dt, rho0, uo, vo, wo, p)
-    divtot_o, divmax_o = oracle.chkdiv(ng, dli, 1, nh_u, dzfi, uo, vo, wo)
+    oracle.fillps(ng, nh_d, nh_u, dli, dzfi, 1.0 / dt, rho0, uo, vo, wo, p)
+    divtot_o, divmax_o = oracle.chkdiv(ng, dli, nh_d, nh_u, dzfi, uo, vo, wo)
     s.updt_rhs_b(p)
     f_o = p.copy(order="F")
     s.solve(p)
-    oracle.boundp(cbc, ng, (0.0,) * 6, 1, dl, dzc, dzf, p)
-    oracle.correc(ng, 1, nh_u, dli, dzci, dt, rho0, p, uo, vo, wo)
+    oracle.boundp(cbc, ng, (0.0,) * 6, nh_d, dl, dzc, dzf, p)
+    oracle.correc(ng, nh_d, nh_u, dli, dzci, dt, rho0, p, uo, vo, wo)
+    oracle.bounduvw(cbcvel, ng, np.zeros(18), nh_d, nh_u, [0] * 6, dl, dzc, dzf, uo, vo, wo)
 
     inner = (slice(1, -1),) * 3
     assert relmax(f_g[inner], f_o[inner]) <= 1e-14

@@ -45,12 +45,14 @@ __global__ void __launch_bounds__(kBatchThreads)
   if (q.tj_half) {
     constexpr int TJ = tj_variant<NC>(1);
     x_fwd_tile<double, double, NC, TJ, kBatchThreads>(
-        q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,
+        q.f, q.hx, q.hxy, q.W, plane_major(q.nx, q.ny), q.ny, q.kx_f, q.twx, q.twhx, q.twqx,
+        nullptr, dyn,
         dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   } else {
     constexpr int TJ = tj_variant<NC>(0);
     x_fwd_tile<double, double, NC, TJ, kBatchThreads>(
-        q.f, q.hx, q.hxy, q.W, q.ny, q.kx_f, q.twx, q.twhx, q.twqx, nullptr, dyn,
+        q.f, q.hx, q.hxy, q.W, plane_major(q.nx, q.ny), q.ny, q.kx_f, q.twx, q.twhx, q.twqx,
+        nullptr, dyn,
         dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   }
 }
@@ -64,12 +66,14 @@ __global__ void __launch_bounds__(kBatchThreads)
   if (q.tj_half) {
     constexpr int TJ = tj_variant<NC>(1);
     x_bwd_tile<double, double, NC, TJ, kBatchThreads, false>(
-        q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,
+        q.W, plane_major(q.nx, q.ny), q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx,
+        q.twqx, dyn,
         dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   } else {
     constexpr int TJ = tj_variant<NC>(0);
     x_bwd_tile<double, double, NC, TJ, kBatchThreads, false>(
-        q.W, q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx, q.twqx, dyn,
+        q.W, plane_major(q.nx, q.ny), q.p, q.hx, q.hxy, q.ny, q.kx_b, q.scale, q.twx, q.twhx,
+        q.twqx, dyn,
         dyn + TJ * line_stride<double, 2 * NC>(), w.u * TJ, w.v);
   }
 }
@@ -77,13 +81,15 @@ __global__ void __launch_bounds__(kBatchThreads)
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_yz(const BatchProblem &q, int i, double *dyn) {
   yz_fwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
-      q.W, q.ck, q.nx, q.nz, q.ky_f, q.lamx, q.lamy, q.a, q.b, q.c, q.pin, q.twy, q.twhy,
+      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.ky_f, q.lamx, q.lamy, q.a, q.b, q.c,
+      q.pin, q.twy, q.twhy,
       q.twqy, dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_zy(const BatchProblem &q, int i, double *dyn) {
   zy_bwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
-      q.W, q.ck, q.nx, q.nz, q.ky_b, q.lamx, q.lamy, q.a, q.b, q.c, q.twy, q.twhy, q.twqy,
+      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.ky_b,
+      q.lamx, q.lamy, q.a, q.b, q.c, q.twy, q.twhy, q.twqy,
       dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }
 

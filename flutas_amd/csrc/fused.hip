@@ -10,52 +10,53 @@ using tile::cx;
 template <typename T, typename Tin, int NC, int TJ, int NT>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_x_fwd(const Tin *__restrict__ in, long long is1, long long is2,
-            T *__restrict__ Wt, int ny, int kind, const T *__restrict__ tw,
+            T *__restrict__ Wt, WStride ws, int ny, int kind, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq,
             double *__restrict__ absmax_part) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
   double *slot = absmax_part ? absmax_part + blockIdx.x + (long long)gridDim.x * blockIdx.y
                              : nullptr;
-  x_fwd_tile<T, Tin, NC, TJ, NT>(in, is1, is2, Wt, ny, kind, tw, twh, twq, slot, sm, stw,
+  x_fwd_tile<T, Tin, NC, TJ, NT>(in, is1, is2, Wt, ws, ny, kind, tw, twh, twq, slot, sm, stw,
                                  blockIdx.x * TJ, blockIdx.y);
 }
 
 template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
-    k_x_bwd(const T *__restrict__ Wt, Tout *__restrict__ out, long long os1,
+    k_x_bwd(const T *__restrict__ Wt, WStride ws, Tout *__restrict__ out, long long os1,
             long long os2, int ny, int kind, T scale, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  x_bwd_tile<T, Tout, NC, TJ, NT, ACC>(Wt, out, os1, os2, ny, kind, scale, tw, twh, twq, sm,
+  x_bwd_tile<T, Tout, NC, TJ, NT, ACC>(Wt, ws, out, os1, os2, ny, kind, scale, tw, twh, twq, sm,
                                        stw, blockIdx.x * TJ, blockIdx.y);
 }
 
 template <typename T, int NC, int GP, int NT, bool PER>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
-    k_yz_fwd(T *__restrict__ Wt, double *__restrict__ ck, int nx, int nz, int kind,
+    k_yz_fwd(T *__restrict__ Wt, WStride ws, double *__restrict__ ck, int nx, int nz, int kind,
              const double *__restrict__ lamx, const double *__restrict__ lamy,
              const double *__restrict__ a, const double *__restrict__ b,
              const double *__restrict__ c, int pin, const T *__restrict__ tw,
              const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  yz_fwd_slab<T, NC, GP, NT, PER>(Wt, ck, nx, nz, kind, lamx, lamy, a, b, c, pin, tw, twh,
+  yz_fwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, kind, lamx, lamy, a, b, c, pin, tw, twh,
                                   twq, sm, stw, blockIdx.x);
 }
 
 template <typename T, int NC, int GP, int NT, bool PER>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
-    k_zy_bwd(T *__restrict__ Wt, const double *__restrict__ ck, int nx, int nz, int kind,
+    k_zy_bwd(T *__restrict__ Wt, WStride ws, const double *__restrict__ ck, int nx, int nz,
+             int kind,
              const double *__restrict__ lamx, const double *__restrict__ lamy,
              const double *__restrict__ a, const double *__restrict__ b,
              const double *__restrict__ c, const T *__restrict__ tw,
              const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  zy_bwd_slab<T, NC, GP, NT, PER>(Wt, ck, nx, nz, kind, lamx, lamy, a, b, c, tw, twh, twq, sm,
-                                  stw, blockIdx.x);
+  zy_bwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, kind, lamx, lamy, a, b, c, tw, twh, twq,
+                                  sm, stw, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host ---
@@ -133,24 +134,24 @@ int build_fused_twiddles(LineTables &tab) {
 
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
-                 long long is2, T *Wt, int ny, int nz, double *absmax_part,
-                 hipStream_t s) {
+                 long long is2, T *Wt, WStride ws, int ny, int nz,
+                 double *absmax_part, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
-                       is2, Wt, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part);
+                       is2, Wt, ws, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
 
 template <typename T, typename Tout>
-int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
-                 long long os1, long long os2, int ny, int nz, T scale,
-                 int accumulate, hipStream_t s) {
+int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
+                 Tout *out, long long os1, long long os2, int ny, int nz,
+                 T scale, int accumulate, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<NC>();
@@ -158,10 +159,10 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
     dim3 grid((ny + TJ - 1) / TJ, nz);
     if (accumulate)
       hipLaunchKernelGGL((k_x_bwd<T, Tout, NC, TJ, NT, true>), grid, dim3(NT), 0, s, Wt,
-                         out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
+                         ws, out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
     else
       hipLaunchKernelGGL((k_x_bwd<T, Tout, NC, TJ, NT, false>), grid, dim3(NT), 0, s, Wt,
-                         out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
+                         ws, out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -201,55 +202,55 @@ int fused_group_planes(int ny, int elem_bytes) {
 }
 
 template <typename T>
-int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, double *ck, int nx,
-                  int nz, const double *lamx, const double *lamy,
-                  const double *a, const double *b, const double *c, int pin,
-                  int periodic, hipStream_t s) {
+int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
+                  double *ck, int nx, int nz, const double *lamx,
+                  const double *lamy, const double *a, const double *b,
+                  const double *c, int pin, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
     constexpr int NTP = slab_threads<T, NC, true>();
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
-      hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ck,
-                         nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+      hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
+                         ck, nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
     else
-      hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ck,
-                         nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+      hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
+                         ck, nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
 
 template <typename T>
-int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
-                  int nx, int nz, const double *lamx, const double *lamy,
-                  const double *a, const double *b, const double *c,
-                  int periodic, hipStream_t s) {
+int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
+                  const double *ck, int nx, int nz, const double *lamx,
+                  const double *lamy, const double *a, const double *b,
+                  const double *c, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
     constexpr int NTP = slab_threads<T, NC, true>();
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
-      hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ck,
-                         nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+      hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
+                         ck, nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
     else
-      hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ck,
-                         nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+      hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
+                         ck, nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
 
-template int launch_x_fwd<double, double>(const LineTables &, int, const double *, long long, long long, double *, int, int, double *, hipStream_t);
-template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, int, int, double *, hipStream_t);
-template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, int, int, double *, hipStream_t);
-template int launch_x_bwd<double, double>(const LineTables &, int, const double *, double *, long long, long long, int, int, double, int, hipStream_t);
-template int launch_x_bwd<float, double>(const LineTables &, int, const float *, double *, long long, long long, int, int, float, int, hipStream_t);
-template int launch_yz_fwd<double>(const LineTables &, int, double *, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
-template int launch_yz_fwd<float>(const LineTables &, int, float *, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
-template int launch_zy_bwd<double>(const LineTables &, int, double *, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
-template int launch_zy_bwd<float>(const LineTables &, int, float *, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
+template int launch_x_fwd<double, double>(const LineTables &, int, const double *, long long, long long, double *, WStride, int, int, double *, hipStream_t);
+template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, WStride, int, int, double *, hipStream_t);
+template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, hipStream_t);
+template int launch_x_bwd<double, double>(const LineTables &, int, const double *, WStride, double *, long long, long long, int, int, double, int, hipStream_t);
+template int launch_x_bwd<float, double>(const LineTables &, int, const float *, WStride, double *, long long, long long, int, int, float, int, hipStream_t);
+template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_yz_fwd<float>(const LineTables &, int, float *, WStride, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_zy_bwd<double>(const LineTables &, int, double *, WStride, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
+template int launch_zy_bwd<float>(const LineTables &, int, float *, WStride, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
 
 }  // namespace flutas

@@ -2,17 +2,21 @@
 // shared by the single-problem kernels (fused.hip) and the batched kernels
 // (batch.hip).
 //
-// Work layout Wt[k][i][j] (j fastest): y-lines are contiguous, and an
-// x-tile of P1/P4 (fixed k) touches a single 2-D plane.
+// The work field W holds (i, j, k) with j fastest (y-lines contiguous) at
+// the strides of a WStride.  The FP64 single solve keeps it in blocks of a
+// few x-slabs (fused_layout_fp64): within a block the planes of the slabs
+// follow each other, so P2/P3 stream nearly contiguous memory while the
+// transposed accesses of P1/P4 stay within few distant regions.  The FP32
+// chain and the batched solves use the plane-major layout.
 //
-//   P1 x_fwd_tile   p (halo layout, x-lines) -> x r2r -> Wt (transposed
+//   P1 x_fwd_tile   p (halo layout, x-lines) -> x r2r -> W (transposed
 //                   store)
 //   P2 yz_fwd_slab  per x-slab i, k ascending: y r2r of each plane, then
 //                   the Thomas forward elimination step for all (i, j)
 //                   columns
 //   P3 zy_bwd_slab  per x-slab i, k descending: Thomas back substitution,
 //                   then the inverse y r2r of each plane
-//   P4 x_bwd_tile   Wt (transposed load) -> inverse x r2r * normfft -> p
+//   P4 x_bwd_tile   W (transposed load) -> inverse x r2r * normfft -> p
 //
 // Each pass reads and writes the field once.  The elimination factors d(k)
 // are not stored: P2 writes one checkpoint d per column every GP planes and
@@ -133,13 +137,14 @@ constexpr int tile_threads() {
 // ---------------------------------------------------------------- P1 -----
 // One workgroup: TJ consecutive x-lines (j0..j0+TJ-1) at one k.
 // in: element (i, j, k) at in[j*is1 + k*is2 + i] (i, j, k 0-based).
-// Wt: element (i, j, k) at Wt[(k*nx + i)*ny + j].
+// Wt: element (i, j, k) at Wt[ws.ioff(i) + k*ws.wk + j] (WStride).
 // The tile body (also used by the batched kernels): sm holds TJ lines of
 // line_stride<T, nx>() elements, stw the staged twiddles; absmax_slot, when
 // not null, receives the tile's max |in|.
 template <typename T, typename Tin, int NC, int TJ, int NT>
 __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long is1,
-                                           long long is2, T *__restrict__ Wt, int ny,
+                                           long long is2, T *__restrict__ Wt, WStride ws,
+                                           int ny,
                                            int kind, const T *__restrict__ tw,
                                            const T *__restrict__ twh,
                                            const T *__restrict__ twq, double *absmax_slot,
@@ -183,19 +188,20 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
   tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
   tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
   // transposed store
-  T *plane = Wt + (long long)k * n * ny;
+  T *plane = Wt + (long long)k * ws.wk;
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int t = threadIdx.x + q * NT;
     const int l = t % TJ, i = t / TJ;
-    if (t < TJ * n && l < nl) plane[(long long)i * ny + j0 + l] = sm[l * LS + i];
+    if (t < TJ * n && l < nl) plane[ws.ioff(i) + j0 + l] = sm[l * LS + i];
   }
 }
 
 // ---------------------------------------------------------------- P4 -----
 // Inverse of P1: out(i, j, k) = scale * r2r^-1 (x) (+ out when ACC).
 template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
-__device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, Tout *__restrict__ out,
+__device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
+                                           Tout *__restrict__ out,
                                            long long os1, long long os2, int ny, int kind,
                                            T scale, const T *__restrict__ tw,
                                            const T *__restrict__ twh,
@@ -206,13 +212,13 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, Tout *__res
   constexpr int LD = (TJ * n + NT - 1) / NT;
   const int nl = min(TJ, ny - j0);
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
-  const T *plane = Wt + (long long)k * n * ny;
+  const T *plane = Wt + (long long)k * ws.wk;
   T r[LD];
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
     const int l = t % TJ, i = t / TJ;
-    r[q] = plane[(long long)i * ny + min(j0 + l, ny - 1)];
+    r[q] = plane[ws.ioff(i) + min(j0 + l, ny - 1)];
   }
   // the previous values of an accumulated output are loaded with the tile
   // so that their latency overlaps the transform
@@ -316,7 +322,8 @@ __host__ __device__ constexpr int checkpoints(int nz) {
 // The slab body (also used by the batched kernels): sm holds 2*GP lines
 // of line_stride<T, ny>() elements, stw the staged twiddles.
 template <typename T, int NC, int GP, int NT, bool PER>
-__device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restrict__ ck,
+__device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
+                                            double *__restrict__ ck,
                                             int nx, int nz, int kind,
                                             const double *__restrict__ lamx,
                                             const double *__restrict__ lamy,
@@ -341,8 +348,8 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
   const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int ngrp = (nz + GP - 1) / GP;
   const int nck = checkpoints<GP>(nz);
-  const long long pstride = (long long)nx * n;  // plane stride
-  T *line0 = Wt + (long long)i * n;             // plane 0 line of slab i
+  const long long pstride = ws.wk;       // plane stride
+  T *line0 = Wt + ws.ioff(i);            // plane 0 line of slab i
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
   const double lx = lamx[i];
   double lam[CPT], dprev[CPT], pprev[CPT];
@@ -499,7 +506,8 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, double *__restri
 
 // ---------------------------------------------------------------- P3 -----
 template <typename T, int NC, int GP, int NT, bool PER>
-__device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__restrict__ ck,
+__device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
+                                            const double *__restrict__ ck,
                                             int nx, int nz, int kind,
                                             const double *__restrict__ lamx,
                                             const double *__restrict__ lamy,
@@ -522,8 +530,8 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, const double *__
   const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
   const int ngrp = (nz + GP - 1) / GP;
   const int nck = checkpoints<GP>(nz);
-  const long long pstride = (long long)nx * n;
-  T *line0 = Wt + (long long)i * n;
+  const long long pstride = ws.wk;
+  T *line0 = Wt + ws.ioff(i);
   const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
   const double lx = lamx[i];
   double lam[CPT], pnext[CPT];

@@ -100,31 +100,61 @@ int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
                            int nz, int pin, hipStream_t s);
 
 // ---- fused four-pass pipeline --------------------------------------------
-// Work layout Wt[k][i][j]; see fused.hip.
+// Layout of a work field, j contiguous: the x-slabs are grouped in blocks
+// of B = 2^lb slabs (B divides nx); element (i, j, k) sits at
+//   W[(i >> lb)*wib + (i & (B-1))*wii + k*wk + j],
+// wib = nz*B*ny, wii = ny, wk = B*ny.  B >= nx is the plane-major layout
+// (an x-tile of P1/P4 at one k touches one 2-D plane), B = 1 the slab-major
+// one (the planes of an x-slab of P2/P3 form one contiguous stream).
+struct WStride {
+  long long wib, wii, wk;
+  int lb;
+  __host__ __device__ long long ioff(int i) const {
+    return (long long)(i >> lb) * wib + (long long)(i & ((1 << lb) - 1)) * wii;
+  }
+};
+// blocks of 2^lb slabs (2^lb must divide nx)
+__host__ __device__ inline WStride slab_blocked(int ny, int nz, int lb) {
+  return {(long long)nz * ny << lb, ny, (long long)ny << lb, lb};
+}
+__host__ __device__ inline WStride plane_major(int nx, int ny) {
+  return {0, ny, (long long)nx * ny, 30};
+}
+// The layout of the FP64 single solve: blocks of 8 slabs, or of the
+// largest power of two dividing nx below that.  Measured at 512^3 on one
+// MI355X: blocks of 4 to 128 slabs run the FP64 solve 3-4% faster than
+// plane-major, blocks of 2 or 1 (slab-major) do not; the FP32 chain of the
+// refined mode is 2% faster plane-major, and the batched solves measure
+// the same either way (they stay plane-major).
+__host__ __device__ inline WStride fused_layout_fp64(int nx, int ny, int nz) {
+  int lb = 0;
+  while (lb < 3 && ((nx >> lb) & 1) == 0) ++lb;
+  return slab_blocked(ny, nz, lb);
+}
 bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
 // Uploads the per-pass twiddle tables of the compiled plan of tab.n.
 int build_fused_twiddles(LineTables &tab);
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
-                 long long is2, T *Wt, int ny, int nz, double *absmax_part,
-                 hipStream_t s);
+                 long long is2, T *Wt, WStride ws, int ny, int nz,
+                 double *absmax_part, hipStream_t s);
 // workgroups of launch_x_fwd (one max|in| partial each when absmax_part)
 int x_fwd_blocks(const LineTables &tx, int elem_bytes, int ny, int nz);
 template <typename T, typename Tout>
-int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, Tout *out,
-                 long long os1, long long os2, int ny, int nz, T scale,
-                 int accumulate, hipStream_t s);
+int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
+                 Tout *out, long long os1, long long os2, int ny, int nz,
+                 T scale, int accumulate, hipStream_t s);
 template <typename T>
-int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, double *ck, int nx,
-                  int nz, const double *lamx, const double *lamy,
-                  const double *a, const double *b, const double *c, int pin,
-                  int periodic, hipStream_t s);
+int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
+                  double *ck, int nx, int nz, const double *lamx,
+                  const double *lamy, const double *a, const double *b,
+                  const double *c, int pin, int periodic, hipStream_t s);
 template <typename T>
-int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, const double *ck,
-                  int nx, int nz, const double *lamx, const double *lamy,
-                  const double *a, const double *b, const double *c,
-                  int periodic, hipStream_t s);
+int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
+                  const double *ck, int nx, int nz, const double *lamx,
+                  const double *lamy, const double *a, const double *b,
+                  const double *c, int periodic, hipStream_t s);
 
 // ---- batched fused pipeline (many independent FP64 problems) ------------
 // One launch per stage and line length covers every problem of that length;

@@ -240,7 +240,8 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
 // One spectral solve in storage/arithmetic T: the right-hand side `in`
 // (halo-1 layout at its interior origin when in_halo, else dense nx*ny*nz)
 // -> out (interior origin of a halo-1 FP64 field): out = S(in), or
-// out += S(in) when accumulate.
+// out += S(in) when accumulate.  The fused FP64 passes keep W in blocks
+// of x-slabs (fused_layout_fp64), the FP32 ones plane-major.
 template <typename T, typename Tin>
 int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
              int accumulate, T *W, hipStream_t s, double *absmax_part = nullptr) {
@@ -252,15 +253,17 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
     // the singular column takes p(last) = 0 for NN and for periodic z
     const int per = (h->cbc[4] == 'P');
     const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
-    FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ny, nz, absmax_part, s)));
+    const WStride ws = sizeof(T) == 8 ? fused_layout_fp64(nx, ny, nz) : plane_major(nx, ny);
+    FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ws, ny, nz, absmax_part,
+                                 s)));
     FL_TRY(mark(h, "P1_x_fwd", s));
-    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
+    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, nx, nz, h->d_lamx, h->d_lamy,
                             h->d_a, h->d_b, h->d_c, pin, per, s));
     FL_TRY(mark(h, "P2_yz_fwd", s));
-    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, h->CK, nx, nz, h->d_lamx, h->d_lamy,
+    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, nx, nz, h->d_lamx, h->d_lamy,
                             h->d_a, h->d_b, h->d_c, per, s));
     FL_TRY(mark(h, "P3_zy_bwd", s));
-    FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, out, q.hx, q.hxy, ny, nz,
+    FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, ws, out, q.hx, q.hxy, ny, nz,
                                     (T)h->normfft, accumulate, s)));
     FL_TRY(mark(h, "P4_x_bwd", s));
     return FLUTAS_OK;

@@ -138,7 +138,7 @@ int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  double *absmax_part, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
   FL_DISPATCH(tx.n / 2, {
-    constexpr int NT = tile_threads<NC>();
+    constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
@@ -154,7 +154,7 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
                  T scale, int accumulate, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
   FL_DISPATCH(tx.n / 2, {
-    constexpr int NT = tile_threads<NC>();
+    constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     if (accumulate)

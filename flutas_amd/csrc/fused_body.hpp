@@ -125,13 +125,16 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 // of two tiles or slabs allows.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
-// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points
-// (measured faster than 256 for the transposed loads of P4), 256 otherwise
-// (faster for the short lines of batched problems; longer lines need more
-// than 128 registers per thread).
-template <int NC>
+// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and for
+// FP64 lines of 1024 points (measured faster than 256: P4 0.49 -> 0.45 ms
+// at 512^3; P1 3.29 -> 2.70 ms and P4 3.91 -> 3.35 ms at 1024x1024x512),
+// 256 otherwise (faster for the short lines of batched problems; the FP32
+// accumulating P4 of 1024-point lines would spill at 128 registers).
+template <typename T, int NC>
 constexpr int tile_threads() {
-  return 2 * NC == 512 ? 512 : 256;
+  if (2 * NC == 512) return 512;
+  if (2 * NC == 1024 && sizeof(T) == 8) return 512;
+  return 256;
 }
 
 // ---------------------------------------------------------------- P1 -----
@@ -703,15 +706,16 @@ constexpr int group_planes() {
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
 
-// Threads of a P2/P3 slab workgroup of a single problem.  512-point
-// y-lines: 512 threads (one column each, at most 128 registers, 4 waves per
-// SIMD) for the non-periodic passes, 256 for the periodic-z passes (whose
-// cyclic state does not fit 128 registers without spilling); 256 for other
-// lengths (a problem with few x-slabs has few workgroups, and more threads
-// shorten each one).
+// Threads of a P2/P3 slab workgroup of a single problem.  512- and
+// 1024-point y-lines: 512 threads (one or two columns each, at most 128
+// registers, 4 waves per SIMD) for the non-periodic passes (at
+// 1024x1024x512: P2 2.99 -> 2.43 ms, P3 3.48 -> 2.63 ms), 256 for the
+// periodic-z passes (whose cyclic state does not fit 128 registers without
+// spilling); 256 for other lengths (a problem with few x-slabs has few
+// workgroups, and more threads shorten each one).
 template <typename T, int NC, bool PER>
 constexpr int slab_threads() {
-  if (2 * NC != 512) return 256;
+  if (2 * NC != 512 && 2 * NC != 1024) return 256;
   return PER ? 256 : 512;
 }
 

@@ -120,13 +120,15 @@ __host__ __device__ inline WStride slab_blocked(int ny, int nz, int lb) {
 __host__ __device__ inline WStride plane_major(int nx, int ny) {
   return {0, ny, (long long)nx * ny, 30};
 }
-// The layout of the FP64 single solve: blocks of 8 slabs, or of the
-// largest power of two dividing nx below that.  Measured at 512^3 on one
-// MI355X: blocks of 4 to 128 slabs run the FP64 solve 3-4% faster than
-// plane-major, blocks of 2 or 1 (slab-major) do not; the FP32 chain of the
-// refined mode is 2% faster plane-major, and the batched solves measure
-// the same either way (they stay plane-major).
+// The layout of the FP64 single solve.  Measured on one MI355X: with
+// 512-point y-lines (4 KB rows; 512^3), blocks of 4 to 128 slabs run the
+// solve 3-4% faster than plane-major and blocks of 2 or 1 (slab-major) do
+// not; with 256- or 1024-point y-lines (256x256x512, 1024x1024x512) blocks
+// of 8 are 1-2% slower, so those stay plane-major.  The FP32 chain of the
+// refined mode is 2% faster plane-major, and the batched solves measure the
+// same either way (they stay plane-major).
 __host__ __device__ inline WStride fused_layout_fp64(int nx, int ny, int nz) {
+  if (ny != 512) return plane_major(nx, ny);
   int lb = 0;
   while (lb < 3 && ((nx >> lb) & 1) == 0) ++lb;
   return slab_blocked(ny, nz, lb);

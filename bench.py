@@ -100,10 +100,12 @@ def cpu_baseline(case, reps=5):
 STEP_SIDE_BYTES = {"reference_order": 128, "fused": 104}
 
 
-def step_bench(api, case, dzc, dzci, dzfi, f_host, steps, warmup):
+def step_bench(api, case, dzc, dzf, dzci, dzfi, f_host, steps, warmup):
     """One projection step around the FP64 solve at the bench size, in the
-    reference's call order and in the fused order with swapped pressure
-    buffers; ms per step from HIP events."""
+    reference's call order (main__single_phase.f90:705-747, bounduvw after
+    correc) and in the fused order with swapped pressure buffers; ms per
+    step from HIP events.  Velocity BCs follow the pressure pattern:
+    periodic stays periodic, a pressure N wall is a no-slip (D, 0) wall."""
     nx, ny, nz = case.ng
     D = api.DeviceArray.from_host
     u, v, w = D(f_host), D(f_host), D(f_host)
@@ -115,6 +117,11 @@ def step_bench(api, case, dzc, dzci, dzfi, f_host, steps, warmup):
     rx, ry, rz = plan.rhsb_device()
     zero_bc = (0.0,) * 6
     dt = 1e-3
+    vel = {"P": "P", "N": "D", "D": "N"}
+    cbcvel = "".join(vel[c] for _ in range(3) for c in case.cbc)
+
+    def bounduvw():
+        api.bounduvw(cbcvel, case.ng, (0.0,) * 18, 1, 1, [0] * 6, dl, dzc, dzf, u, v, w)
 
     def reference_order():
         api.copy_interior(nx, ny, nz, p, pold)
@@ -123,6 +130,7 @@ def step_bench(api, case, dzc, dzci, dzfi, f_host, steps, warmup):
         plan.solve_async(p)
         api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
         api.correc(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0, p, u, v, w)
+        bounduvw()
         api.add_interior(nx, ny, nz, pold, p)
         api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
 
@@ -133,6 +141,7 @@ def step_bench(api, case, dzc, dzci, dzfi, f_host, steps, warmup):
         api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], scratch)
         api.correc_update(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0,
                           scratch, p, p, u, v, w)
+        bounduvw()
         api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
 
     t_ref = time_steps(reference_order, steps, warmup, api)
@@ -316,7 +325,7 @@ def main():
     }
     if not args.no_step:
         log("bench: projection step (reference order vs fused) ...")
-        result["projection_step"] = step_bench(api, case, dzc, dzci, dzfi, f_host,
+        result["projection_step"] = step_bench(api, case, dzc, dzf, dzci, dzfi, f_host,
                                                max(3, args.steps // 2), args.warmup)
         st = result["projection_step"]
         log(f"bench: step {st['reference_order_ms']:.3f} ms in the reference order, "

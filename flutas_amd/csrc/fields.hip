@@ -887,6 +887,9 @@ int flutas_bounduvw(const char cbc[18], const int n[3], const double bc[18],
       FaceArgs a = face(d, side);
       for (int c = 0; c < 3; ++c) {
         if (d > 0 && CB(side, d, c) == 'P') continue;  // done by updthalo
+        // a periodic set_bc copies both sides; its second call (side 1)
+        // would repeat the same copies
+        if (d == 0 && side == 1 && CB(side, d, c) == 'P') continue;
         FaceField &F = a.f[c];
         F.p = fld[c];
         F.ctype = CB(side, d, c);
@@ -984,7 +987,7 @@ int flutas_boundp_halo(const char cbc[6], const int n[3], const double bc[6],
   if (cbc[2] == 'P') FL_TRY(run(1, 0, 'P', 0.0, 0.0, false, false));
   if (cbc[4] == 'P') FL_TRY(run(2, 0, 'P', 0.0, 0.0, false, false));
   FL_TRY(run(0, 0, cbc[0], bc[0], dl[0], false, false));
-  FL_TRY(run(0, 1, cbc[1], bc[1], dl[0], false, false));
+  if (cbc[1] != 'P') FL_TRY(run(0, 1, cbc[1], bc[1], dl[0], false, false));  // P: both sides done
   if (cbc[2] != 'P') FL_TRY(run(1, 0, cbc[2], bc[2], dl[1], false, false));
   if (cbc[3] != 'P') FL_TRY(run(1, 1, cbc[3], bc[3], dl[1], false, false));
   if (cbc[4] != 'P') FL_TRY(run(2, 0, cbc[4], bc[4], 0.0, true, false));

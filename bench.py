@@ -69,7 +69,9 @@ def time_steps(fn, steps, warmup, api):
 
 def cpu_baseline(case, reps=5):
     """The C FP64 oracle (solver_cpu restated, own FFT, OpenMP) on the same
-    512^3 workload, timed over `reps` solves; ns per grid point (median)."""
+    512^3 workload, timed over `reps` solves; ns per grid point (median).
+    Also returns the oracle's solution (interior), the reference the GPU
+    results are compared with."""
     from oracle import pyoracle as po
     from flutas_amd import workloads as wl
     nx, ny, nz = case.ng
@@ -84,11 +86,20 @@ def cpu_baseline(case, reps=5):
         times.append(time.perf_counter() - t0)
     npts = nx * ny * nz
     med = float(np.median(times))
-    return {"value": med * 1e9 / npts, "unit": "ns/gridpoint",
-            "cores": int(po.lib().or_max_threads()), "kind": "port",
-            "best": min(times) * 1e9 / npts,
-            "sample": f"C FP64 oracle (solver_cpu restated, own FFT, OpenMP) on the same "
-                      f"{nx}x{ny}x{nz} {case.cbc} stretch-{case.alpha} solve; median of {reps}"}
+    return ({"value": med * 1e9 / npts, "unit": "ns/gridpoint",
+             "cores": int(po.lib().or_max_threads()), "kind": "port",
+             "best": min(times) * 1e9 / npts,
+             "sample": f"C FP64 oracle (solver_cpu restated, own FFT, OpenMP) on the same "
+                       f"{nx}x{ny}x{nz} {case.cbc} stretch-{case.alpha} solve; median of {reps}"},
+            p[1:-1, 1:-1, 1:-1])
+
+
+def relmax(a, b, gauge_free=False):
+    """max|a - b| / max|b|; gauge_free removes the mean of both first (the
+    additive constant of a singular problem)."""
+    if gauge_free:
+        a, b = a - a.mean(), b - b.mean()
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
 # Algorithmic bytes per grid point of the field passes around the solve in
@@ -250,6 +261,7 @@ def main():
     t64 = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
     ms = t64["mean"]
     relres64 = plan.residual(f, p)
+    sol64 = p.to_host()[1:-1, 1:-1, 1:-1]
     log(f"bench: fp64 {ms:.3f} ms/solve (median {t64['median']:.3f}, min {t64['min']:.3f}, "
         f"wall {t64['wall']:.3f}), relres {relres64:.2e}")
     # per-stage kernel times: HIP events between the stages on the library
@@ -290,6 +302,8 @@ def main():
                   "stages_ms": [[s, t] for s, t in stages32],
                   "algorithmic_bytes_per_point": 48 + 96 * iters + 24}
         log(f"bench: fp32+refine {ms32:.3f} ms/solve, {iters} iters, relres {rel:.2e}")
+        plan32.solve_into(f, p)
+        sol32 = p.to_host()[1:-1, 1:-1, 1:-1]
         plan32.destroy()
 
     result = {
@@ -313,6 +327,9 @@ def main():
                                "(dz-weighted mean removed), inputs resident in HBM",
                    "ng": list(case.ng), "bc": case.cbc},
         "roofline": {"bound": "hbm", "kernel": dom["stage"],
+                     "time_source": "HIP events recorded between the pipeline stages on the "
+                                    "library stream over back-to-back solves "
+                                    "(flutas_psolve_profile)",
                      "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": STAGE_BYTES_FP64 * npts,
@@ -349,7 +366,17 @@ def main():
         log(f"bench: calibration copy of {COPY_BYTES} B (interior of {COPY_N + 2}^3)")
     if not args.no_cpu:
         log("bench: cpu baseline (oracle) ...")
-        result["cpu_baseline"] = cpu_baseline(case)
+        result["cpu_baseline"], ref = cpu_baseline(case)
+        # the north star's "FP64-oracle error per result": relative max-norm
+        # differences from the oracle's solution (same pinned gauge for
+        # FP64; gauge-free for the refined solve)
+        result["fp64_oracle_err"] = relmax(sol64, ref)
+        if refine is not None:
+            refine["oracle_err"] = relmax(sol32, ref, gauge_free=True)
+            result["refine_oracle_err"] = refine["oracle_err"]
+        log(f"bench: oracle error fp64 {result['fp64_oracle_err']:.2e}" +
+            (f", refined {refine['oracle_err']:.2e}" if refine is not None else ""))
+        del ref
         log(f"bench: cpu {result['cpu_baseline']['value']:.2f} ns/pt on "
             f"{result['cpu_baseline']['cores']} threads")
     print(json.dumps(result), flush=True)

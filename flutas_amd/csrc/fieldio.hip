@@ -183,25 +183,31 @@ int flutas_load_scalar(char io, const char *filename, double *time, int *istep, 
   }
   FILE *fh = std::fopen(filename, "w");
   if (!fh) return fail(FLUTAS_EINVAL, std::string("cannot open ") + filename);
-  // Fortran '(2E15.7, 1I9.8)': two E15.7 fields (0.dddddddE+xx), then I9.8
+  // Fortran '(2E15.7, 1I9.8)': two E15.7 fields, then I9.8.  Ew.d prints
+  // 0.dddddddE+xx, and drops the E for exponents of three digits
+  // (0.ddddddd+xxx).  The seven digits are those of one correctly rounded
+  // %.6E conversion (d.dddddd x 10^e), moved behind the point in the text
+  // (exponent e + 1), so no second rounding can change them.
   char buf[2][32];
   for (int q = 0; q < 2; ++q) {
     const double x = (q == 0) ? *time : *dto;
-    // %.6E gives d.ddddddE+xx; E15.7 wants 0.dddddddE+xx: shift by one decade
+    char digits[8] = "0000000";
     int ex = 0;
-    double m = 0.0;
     if (x != 0.0) {
-      char t[32];
-      std::snprintf(t, sizeof t, "%.6E", x);
+      char t[40];
+      std::snprintf(t, sizeof t, "%.6E", std::fabs(x));  // "d.ddddddE+xx"
+      digits[0] = t[0];
+      std::memcpy(digits + 1, t + 2, 6);
       std::sscanf(std::strchr(t, 'E') + 1, "%d", &ex);
       ex += 1;
-      m = std::fabs(x) / std::pow(10.0, ex);
-      if (m >= 0.99999995) { m /= 10.0; ex += 1; }
     }
-    char digits[16];
-    std::snprintf(digits, sizeof digits, "%.7f", m);  // "0.ddddddd"
-    std::snprintf(buf[q], sizeof buf[q], "%s%sE%c%02d", x < 0 ? "-" : "", digits,
-                  ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    const int ax = ex < 0 ? -ex : ex;
+    if (ax <= 99)
+      std::snprintf(buf[q], sizeof buf[q], "%s0.%sE%c%02d", x < 0 ? "-" : "", digits,
+                    ex < 0 ? '-' : '+', ax);
+    else
+      std::snprintf(buf[q], sizeof buf[q], "%s0.%s%c%03d", x < 0 ? "-" : "", digits,
+                    ex < 0 ? '-' : '+', ax);
   }
   std::fprintf(fh, "%15s%15s%9.8d\n", buf[0], buf[1], *istep);
   std::fclose(fh);

@@ -597,6 +597,18 @@ __global__ void __launch_bounds__(256)
     part[blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z)] = smax[0];
 }
 
+// flag = seq where the interior of the halo-1 field f holds a NaN or an
+// infinity (the five-pass line pipeline, whose x transform has no check of
+// its own)
+__global__ void __launch_bounds__(256)
+    k_flag_nonfinite(const double *__restrict__ f, int nx, int ny, int *flag, int seq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= nx) return;
+  const double v = f[(i + 1) + (long long)(nx + 2) * ((j + 1) + (long long)(ny + 2) * (k + 1))];
+  if (!(fabs(v) <= 1.7976931348623157e308)) *flag = seq;
+}
+
 __global__ void __launch_bounds__(256)
     k_accumulate(const float *__restrict__ r, double *__restrict__ p, int nx,
                  int ny) {
@@ -682,6 +694,15 @@ int launch_absmax(const double *f, int nx, int ny, int nz, double *partial,
   hipLaunchKernelGGL(k_absmax, grid, dim3(bx), 0, s, f, nx, ny, nz, partial);
   FL_HIP(hipGetLastError());
   *nparts = (int)(grid.x * grid.y * grid.z);
+  return FLUTAS_OK;
+}
+
+int launch_flag_nonfinite(const double *f, int nx, int ny, int nz, int *flag, int seq,
+                          hipStream_t s) {
+  const int bx = 256;
+  hipLaunchKernelGGL(k_flag_nonfinite, dim3((nx + bx - 1) / bx, ny, nz), dim3(bx), 0, s, f, nx,
+                     ny, flag, seq);
+  FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
 

@@ -12,13 +12,13 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_x_fwd(const Tin *__restrict__ in, long long is1, long long is2,
             T *__restrict__ Wt, WStride ws, int ny, int kind, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq,
-            double *__restrict__ absmax_part) {
+            double *__restrict__ absmax_part, int *nf_flag, int nf_seq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
   double *slot = absmax_part ? absmax_part + blockIdx.x + (long long)gridDim.x * blockIdx.y
                              : nullptr;
   x_fwd_tile<T, Tin, NC, TJ, NT>(in, is1, is2, Wt, ws, ny, kind, tw, twh, twq, slot, sm, stw,
-                                 blockIdx.x * TJ, blockIdx.y);
+                                 blockIdx.x * TJ, blockIdx.y, nf_flag, nf_seq);
 }
 
 template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
@@ -135,14 +135,15 @@ int build_fused_twiddles(LineTables &tab) {
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  long long is2, T *Wt, WStride ws, int ny, int nz,
-                 double *absmax_part, hipStream_t s) {
+                 double *absmax_part, int *nf_flag, int nf_seq, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid((ny + TJ - 1) / TJ, nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
-                       is2, Wt, ws, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part);
+                       is2, Wt, ws, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part, nf_flag,
+                       nf_seq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -243,9 +244,9 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
   return FLUTAS_OK;
 }
 
-template int launch_x_fwd<double, double>(const LineTables &, int, const double *, long long, long long, double *, WStride, int, int, double *, hipStream_t);
-template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, WStride, int, int, double *, hipStream_t);
-template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, hipStream_t);
+template int launch_x_fwd<double, double>(const LineTables &, int, const double *, long long, long long, double *, WStride, int, int, double *, int *, int, hipStream_t);
+template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
+template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_bwd<double, double>(const LineTables &, int, const double *, WStride, double *, long long, long long, int, int, double, int, hipStream_t);
 template int launch_x_bwd<float, double>(const LineTables &, int, const float *, WStride, double *, long long, long long, int, int, float, int, hipStream_t);
 template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);

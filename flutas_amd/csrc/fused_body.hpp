@@ -143,7 +143,9 @@ constexpr int tile_threads() {
 // Wt: element (i, j, k) at Wt[ws.ioff(i) + k*ws.wk + j] (WStride).
 // The tile body (also used by the batched kernels): sm holds TJ lines of
 // line_stride<T, nx>() elements, stw the staged twiddles; absmax_slot, when
-// not null, receives the tile's max |in|.
+// not null, receives the tile's max |in|; nf_flag, when not null, receives
+// nf_seq from every lane that loads a NaN or an infinity (a plain store of
+// the same value, no ordering needed).
 template <typename T, typename Tin, int NC, int TJ, int NT>
 __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long is1,
                                            long long is2, T *__restrict__ Wt, WStride ws,
@@ -151,7 +153,8 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
                                            int kind, const T *__restrict__ tw,
                                            const T *__restrict__ twh,
                                            const T *__restrict__ twq, double *absmax_slot,
-                                           T *sm, T *stw, int j0, int k) {
+                                           T *sm, T *stw, int j0, int k,
+                                           int *nf_flag = nullptr, int nf_seq = 0) {
   constexpr int n = 2 * NC;  // nx
   constexpr int LS = line_stride<T, n>();
   constexpr int LD = (TJ * n + NT - 1) / NT;
@@ -186,6 +189,12 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
 #pragma unroll
     for (int q = 0; q < LD; ++q) am = nan_max(am, (double)r[q]);
     block_max_store<NT>(am, absmax_slot);
+  }
+  if (nf_flag) {
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < LD; ++q) bad |= !(fabs((double)r[q]) <= 1.7976931348623157e308);
+    if (bad) *nf_flag = nf_seq;
   }
   __syncthreads();
   tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);

@@ -137,10 +137,12 @@ bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
 // Uploads the per-pass twiddle tables of the compiled plan of tab.n.
 int build_fused_twiddles(LineTables &tab);
+// absmax_part: one max|in| partial per workgroup (or null); nf_flag: set
+// to nf_seq when the input holds a NaN or an infinity (or null).
 template <typename T, typename Tin>
 int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  long long is2, T *Wt, WStride ws, int ny, int nz,
-                 double *absmax_part, hipStream_t s);
+                 double *absmax_part, int *nf_flag, int nf_seq, hipStream_t s);
 // workgroups of launch_x_fwd (one max|in| partial each when absmax_part)
 int x_fwd_blocks(const LineTables &tx, int elem_bytes, int ny, int nz);
 template <typename T, typename Tout>
@@ -213,6 +215,10 @@ int launch_accumulate(const float *r, double *p_halo, int nx, int ny, int nz,
                       hipStream_t s);
 int launch_absmax(const double *f_halo, int nx, int ny, int nz, double *partial,
                   int *nparts, hipStream_t s);
+// flag = seq when the interior of the halo-1 field f holds a NaN or an
+// infinity (flag untouched otherwise)
+int launch_flag_nonfinite(const double *f_halo, int nx, int ny, int nz, int *flag, int seq,
+                          hipStream_t s);
 // partial count of launch_residual and launch_absmax
 size_t residual_blocks(int nx, int ny, int nz);
 // out[0] = max of n partial maxima (single workgroup)

@@ -132,6 +132,12 @@ struct flutas_psolve {
   double *CK = nullptr;    // z elimination checkpoints of the fused pipeline
   int last_iters = 0;
   double last_relres = -1.0;
+  // NaN/Inf detection of the FP64 solve: P1 (or a flag pass on the line
+  // pipeline) stores the solve's sequence number into a host-mapped word
+  // when the right-hand side holds a non-finite value; the synchronous
+  // entry points compare it after their stream synchronisation.
+  int *nf_host = nullptr, *nf_dev = nullptr;
+  int seq = 0;
   // stage timing (flutas_psolve_profile)
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -158,6 +164,8 @@ void release(flutas_psolve *h) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
+  if (h->nf_host) (void)hipHostFree(h->nf_host);
+  h->nf_host = h->nf_dev = nullptr;
   h->tx.release();
   h->ty.release();
 }
@@ -244,7 +252,8 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
 // of x-slabs (fused_layout_fp64), the FP32 ones plane-major.
 template <typename T, typename Tin>
 int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
-             int accumulate, T *W, hipStream_t s, double *absmax_part = nullptr) {
+             int accumulate, T *W, hipStream_t s, double *absmax_part = nullptr,
+             int *nf_flag = nullptr) {
   const Geo q = halo_geo(h);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   const long long is1 = in_halo ? q.hx : nx;
@@ -255,7 +264,7 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
     const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
     const WStride ws = sizeof(T) == 8 ? fused_layout_fp64(nx, ny, nz) : plane_major(nx, ny);
     FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ws, ny, nz, absmax_part,
-                                 s)));
+                                 nf_flag, h->seq, s)));
     FL_TRY(mark(h, "P1_x_fwd", s));
     FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, nx, nz, h->d_lamx, h->d_lamy,
                             h->d_a, h->d_b, h->d_c, pin, per, s));
@@ -294,7 +303,10 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
 
 int solve_fp64(flutas_psolve *h, const double *f, double *p, hipStream_t s) {
   const Geo q = halo_geo(h);
-  FL_TRY((spectral<double, double>(h, f + q.interior, true, p + q.interior, 0, h->W64, s)));
+  ++h->seq;
+  if (!h->fused) FL_TRY(launch_flag_nonfinite(f, h->nx, h->ny, h->nz, h->nf_dev, h->seq, s));
+  FL_TRY((spectral<double, double>(h, f + q.interior, true, p + q.interior, 0, h->W64, s,
+                                   nullptr, h->fused ? h->nf_dev : nullptr)));
   h->last_iters = 0;
   h->last_relres = -1.0;
   return FLUTAS_OK;
@@ -528,6 +540,11 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
       if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
       if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
     }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&h->nf_host, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) {
+      *h->nf_host = 0;
+      e = hipHostGetDevicePointer((void **)&h->nf_dev, h->nf_host, 0);
+    }
     h->nres = residual_blocks(h->nx, h->ny, h->nz);
     h->nx1 = (h->fused && precision == FLUTAS_FP32_REFINE) ? (size_t)x_fwd_blocks(h->tx, 4, h->ny, h->nz) : 0;
     if (e == hipSuccess) e = hipMalloc((void **)&h->part, (h->nres + h->nx1 + 2) * sizeof(double));
@@ -585,16 +602,25 @@ static int solve_impl(flutas_psolve *h, const double *f, double *p) {
 
 int flutas_psolve_solve_async(flutas_psolve *h, double *p) { return solve_impl(h, p, p); }
 
+// After the stream synchronisation of a synchronous FP64 solve: a
+// non-finite right-hand side was seen when the flag holds this solve's
+// sequence number.
+static int check_nonfinite(const flutas_psolve *h) {
+  if (h->precision == FLUTAS_FP64 && *(volatile int *)h->nf_host == h->seq)
+    return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+  return FLUTAS_OK;
+}
+
 int flutas_psolve_solve_into(flutas_psolve *h, const double *f, double *p) {
   FL_TRY(solve_impl(h, f, p));
   FL_HIP(hipStreamSynchronize(current_stream()));
-  return FLUTAS_OK;
+  return check_nonfinite(h);
 }
 
 int flutas_psolve_solve(flutas_psolve *h, double *p) {
   FL_TRY(flutas_psolve_solve_async(h, p));
   FL_HIP(hipStreamSynchronize(current_stream()));
-  return FLUTAS_OK;
+  return check_nonfinite(h);
 }
 
 int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,

@@ -5,7 +5,8 @@
 ! Fields that are HIP device or managed allocations go to the kernels as
 ! they are; plain host arrays are staged through device buffers
 ! (flutas_stage).  As in the reference, _CONSTANT_COEFFS_POISSON selects the
-! constant-coefficient branches of correc and pres_tw_src.
+! constant-coefficient branches of fillps (the rho0 factor), correc and
+! pres_tw_src.
 module flutas_field_sizes
   use, intrinsic :: iso_c_binding
   implicit none
@@ -39,12 +40,21 @@ contains
     real(rp), intent(in ), dimension(1-nh_u:,1-nh_u:,1-nh_u:), target, contiguous :: u,v,w
     real(rp), intent(out), dimension(0:,0:,0:), target, contiguous :: p
     type(c_ptr) :: du,dv,dw,dp,dz
+    real(rp) :: scale
+    ! the kernel multiplies by its rho0 argument; the reference scales by
+    ! rho0 only under _CONSTANT_COEFFS_POISSON (src/fillps.f90:55-57), and
+    ! a product with 1 is exact
+#if defined(_CONSTANT_COEFFS_POISSON)
+    scale = rho0
+#else
+    scale = 1._rp
+#endif
     du = stage_in(1, c_loc(u), nbytes(size(u)), .true.)
     dv = stage_in(2, c_loc(v), nbytes(size(v)), .true.)
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dp = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
     dz = stage_in(5, c_loc(dzfi), nbytes(size(dzfi)), .true.)
-    call flutas_check(flutas_fillps(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dti,rho0,du,dv,dw,dp), 'fillps')
+    call flutas_check(flutas_fillps(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dti,scale,du,dv,dw,dp), 'fillps')
     call stage_out(c_loc(p), dp, nbytes(size(p)))
   end subroutine fillps
 end module mod_fillps

@@ -38,7 +38,13 @@ contains
       dsize(slot) = bytes
     end if
     d = dbuf(slot)
-    if (copy) call flutas_check(flutas_memcpy_h2d(d, host, bytes), 'host to device copy')
+    ! the blocking copy runs on the null stream; a kernel still reading this
+    ! slot from an earlier call may be queued on the library stream (which
+    ! flutas_set_stream may have made non-blocking), so wait for it first
+    if (copy) then
+      call flutas_check(flutas_synchronize(), 'synchronize')
+      call flutas_check(flutas_memcpy_h2d(d, host, bytes), 'host to device copy')
+    end if
   end function stage_in
   !
   ! Copies a staged output back to the host array (no-op when d is host).

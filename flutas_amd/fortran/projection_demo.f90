@@ -7,7 +7,7 @@
 ! pattern: periodic stays periodic, a pressure N wall is a no-slip (D, 0)
 ! wall, a pressure D face a zero-gradient (N) face, no outflow faces.
 !
-!   projection_demo INPUT OUTPUT MODE
+!   projection_demo INPUT OUTPUT MODE [SOLVER]
 !
 ! INPUT (stream binary): int32 nx,ny,nz,nh_u; float64 lx,ly,lz,alpha,dt,rho0;
 ! 6 characters cbc (the (0:1,3) order); then u, v, w as
@@ -16,20 +16,26 @@
 ! the solve and boundp), u, v, w (after correc and bounduvw), divtot and
 ! divmax of the predicted velocity.
 ! MODE: "host" passes plain Fortran arrays (the modules stage them through
-! device buffers); "device" passes HIP device allocations.
+! device buffers); "device" passes HIP device allocations.  SOLVER: "gpu"
+! (default) calls solver_gpu, "cpu" the drop-in solver_cpu
+! (src/solver_cpu.f90:20-31), as the reference's CPU builds do.
+! Built twice: with _CONSTANT_COEFFS_POISSON (projection_demo) and without
+! (projection_demo_vc: fillps without rho0, correc dividing by the face
+! density of rho = rho0).
 program projection_demo
   use, intrinsic :: iso_c_binding
   use flutas_capi
   use mod_initsolver, only: initsolver
   use mod_fft,        only: fftend
   use mod_solver_gpu, only: solver_gpu
+  use mod_solver_cpu, only: solver_cpu
   use mod_fillps,     only: fillps
   use mod_correc,     only: correc
   use mod_chkdiv,     only: chkdiv
   use mod_bound,      only: bounduvw, boundp, updt_rhs_b
   implicit none
   integer, parameter :: rp = c_double
-  character(len=512) :: fin, fout, mode
+  character(len=512) :: fin, fout, mode, solver
   integer(c_int32_t) :: hdr(4)
   real(rp) :: par(6)
   character(len=1) :: cbc(0:1,3), cbc6(6), cbcvel(0:1,3,3)
@@ -44,6 +50,8 @@ program projection_demo
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   call get_command_argument(3, mode)
+  solver = 'gpu'
+  if (command_argument_count() >= 4) call get_command_argument(4, solver)
   open(newunit=u_in, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u_in) hdr, par, cbc6
   n = hdr(1:3)
@@ -129,7 +137,11 @@ contains
     call chkdiv(n(1), n(2), n(3), dli(1), dli(2), dli(3), nh_d, nh_u, dzfi, uu, vv, ww, divtot, divmax)
     call updt_rhs_b(n(1), n(2), n(3), ['c','c','c'], cbc, 1, rhsbx, rhsby, rhsbz, pp)
     call snapshot(pp, f)
-    call solver_gpu(n, [1,1,1], arrplan, normfft, lambdaxy, a, b, c, cbc, ['c','c','c'], pp)
+    if (trim(solver) == 'cpu') then
+      call solver_cpu(n, arrplan, normfft, lambdaxy, a, b, c, cbc(:,3), ['c','c','c'], pp)
+    else
+      call solver_gpu(n, [1,1,1], arrplan, normfft, lambdaxy, a, b, c, cbc, ['c','c','c'], pp)
+    end if
     call boundp(cbc, n, bc, nh_d, 1, [1,1,1], dl, dzc, dzf, pp)
     call correc(n(1), n(2), n(3), nh_d, nh_u, dli(1), dli(2), dli(3), dzci, dt, rho0, pp, uu, vv, ww, rho)
     call bounduvw(cbcvel, n, bcvel, nh_d, nh_u, [1,1,1], no_outflow, dl, dzc, dzf, uu, vv, ww)

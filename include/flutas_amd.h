@@ -92,11 +92,14 @@ int flutas_psolve_rhsb_device(const flutas_psolve *h, const double **rhsbx,
 /* solver_gpu, one-GPU branch (src/solver_gpu.f90:31-47, :107-229): solves
  * the Poisson equation in place on the interior of the device array p
  * (0:nx+1,0:ny+1,0:nz+1); halo values are neither read nor written.
- * Returns when the result is in p. */
+ * Returns when the result is in p.  A right-hand side holding NaN or Inf
+ * returns FLUTAS_ENONFINITE (p then holds non-finite values). */
 int flutas_psolve_solve(flutas_psolve *h, double *p);
 
 /* Same, enqueued on the library stream; returns immediately.  The refined
- * mode still synchronises once per refinement iteration (residual norm). */
+ * mode still synchronises once per refinement iteration (residual norm)
+ * and reports a non-finite right-hand side; the FP64 mode does not check
+ * here (the synchronous entry points do). */
 int flutas_psolve_solve_async(flutas_psolve *h, double *p);
 
 /* Out-of-place variant (no counterpart in the reference, whose solver works

@@ -159,6 +159,12 @@ def test_load_scalar_reference_format(tmp_path):
     assert f.read_text() == " -0.1234568E+03  0.0000000E+00 00012345\n"
     t, it, dto = api.load_scalar("r", f)
     assert (t, it, dto) == (-123.4568, 12345, 0.0)
+    # rounding that carries into the exponent, and three-digit exponents,
+    # which Ew.d writes without the E
+    api.load_scalar("w", f, 9.99999996, 1, 1.5e-120)
+    assert f.read_text() == "  0.1000000E+02  0.1500000-119 00000001\n"
+    api.load_scalar("w", f, -2.5e100, 3, 0.123456749999)
+    assert f.read_text() == " -0.2500000+101  0.1234567E+00 00000003\n"
     # a list-directed line with commas, as a hand-edited restart might hold
     f.write_text("2.5, 1.0E-4, 7\n")
     assert api.load_scalar("r", f) == (2.5, 7, 1e-4)

@@ -40,19 +40,29 @@ def _velocity(ng, h, cbc, rng):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cbc,alpha,nh_u,mode", [
-    ("PPPPNN", 1.5, 1, "host"),
-    ("PPPPNN", 1.5, 1, "device"),
-    ("PPPPDD", 0.0, 2, "host"),
-    ("NNPPNN", 1.2, 3, "device"),
+@pytest.mark.parametrize("cbc,alpha,nh_u,mode,solver,const_coeff", [
+    ("PPPPNN", 1.5, 1, "host", "gpu", True),
+    ("PPPPNN", 1.5, 1, "device", "gpu", True),
+    ("PPPPDD", 0.0, 2, "host", "gpu", True),
+    ("NNPPNN", 1.2, 3, "device", "gpu", True),
+    # mod_solver_cpu (src/solver_cpu.f90:20-31) backed by the same plan
+    ("PPPPNN", 1.5, 1, "host", "cpu", True),
+    ("DDPPNN", 0.0, 1, "device", "cpu", True),
+    # modules built without _CONSTANT_COEFFS_POISSON: fillps without the
+    # rho0 factor (src/fillps.f90:55-57), correc with the face density
+    ("PPPPNN", 1.5, 1, "host", "gpu", False),
+    ("NNPPNN", 0.0, 2, "device", "gpu", False),
 ])
-def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
-    if not os.path.exists(DEMO):
-        pytest.fail("projection_demo is not built (make -C flutas_amd/fortran)")
+def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode, solver,
+                                 const_coeff):
+    demo = DEMO if const_coeff else DEMO + "_vc"
+    if not os.path.exists(demo):
+        pytest.fail(f"{os.path.basename(demo)} is not built (make -C flutas_amd/fortran)")
     ng = (32, 24, 16)
     nx, ny, nz = ng
     lengths = (2 * np.pi, np.pi, 1.0)
-    dt, rho0 = 0.01, 1.0
+    # rho0 != 1 so that the rho0 factor of fillps is visible
+    dt, rho0 = 0.01, 2.5
     rng = np.random.default_rng(5)
     u, v, w = _velocity(ng, nh_u, cbc, rng)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
@@ -62,9 +72,10 @@ def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
         fh.write(cbc.encode())
         for a in (u, v, w):
             fh.write(a.tobytes(order="F"))
-    print(f"fortran step {cbc} alpha={alpha} nh_u={nh_u} mode={mode}", flush=True)
-    r = subprocess.run([DEMO, str(fin), str(fout), mode], capture_output=True, text=True,
-                       timeout=100)
+    print(f"fortran step {cbc} alpha={alpha} nh_u={nh_u} mode={mode} solver={solver} "
+          f"const_coeff={const_coeff}", flush=True)
+    r = subprocess.run([demo, str(fin), str(fout), mode, solver], capture_output=True,
+                       text=True, timeout=100)
     assert r.returncode == 0, r.stdout + r.stderr
     # chkdiv reports as the reference does (src/chkdiv.f90:66)
     assert "Total divergence = " in r.stdout and "| Maximum divergence = " in r.stdout
@@ -90,13 +101,16 @@ def test_fortran_projection_step(oracle, tmp_path, cbc, alpha, nh_u, mode):
     s = oracle.Solver(ng, dli, dzci, dzfi, cbc, nh_d=nh_d)
     p = np.zeros(hp, order="F")
     uo, vo, wo = (a.copy(order="F") for a in (u, v, w))
-    oracle.fillps(ng, nh_d, nh_u, dli, dzfi, 1.0 / dt, rho0, uo, vo, wo, p)
+    oracle.fillps(ng, nh_d, nh_u, dli, dzfi, 1.0 / dt, rho0 if const_coeff else 1.0,
+                  uo, vo, wo, p)
     divtot_o, divmax_o = oracle.chkdiv(ng, dli, nh_d, nh_u, dzfi, uo, vo, wo)
     s.updt_rhs_b(p)
     f_o = p.copy(order="F")
     s.solve(p)
     oracle.boundp(cbc, ng, (0.0,) * 6, nh_d, dl, dzc, dzf, p)
-    oracle.correc(ng, nh_d, nh_u, dli, dzci, dt, rho0, p, uo, vo, wo)
+    rho = np.full(hp, rho0, order="F")
+    oracle.correc(ng, nh_d, nh_u, dli, dzci, dt, rho0, p, uo, vo, wo, rho=rho,
+                  const_coeff=const_coeff)
     oracle.bounduvw(cbcvel, ng, np.zeros(18), nh_d, nh_u, [0] * 6, dl, dzc, dzf, uo, vo, wo)
 
     inner = (slice(1, -1),) * 3

@@ -94,6 +94,13 @@ SOLVE_CASES = [
     ((6, 4, 2), (1.0, 1.0, 1.0), "PPNNNN", 0.0),
     ((768, 32, 6), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
     ((32, 1024, 4), (1.0, 1.0, 1.0), "NNPPNN", 0.0),
+    # 1024-point x-lines (512-thread FP64 tiles) with R2HC and RODFT kinds,
+    # several periodic-z plane groups with 1024-point y-lines, and
+    # 2048-point x-lines (the largest compiled plan)
+    ((1024, 32, 6), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((1024, 32, 6), (1.0, 1.0, 1.0), "DDPPNN", 0.0),
+    ((32, 1024, 12), (1.0, 1.0, 1.0), "PPPPPP", 0.0),
+    ((2048, 32, 4), (1.0, 1.0, 1.0), "NNPPNN", 0.0),
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
@@ -336,3 +343,27 @@ def test_refined_rejects_nonfinite_rhs(api, oracle):
     with pytest.raises(api.FlutasError) as e:
         plan.solve(api.DeviceArray.from_host(f))
     assert e.value.code == api.FLUTAS_ENONFINITE
+
+
+@pytest.mark.parametrize("unfused", ["0", "1"])
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_fp64_solve_reports_nonfinite(api, oracle, monkeypatch, unfused, bad):
+    """A NaN or an infinity in the right-hand side makes the synchronous FP64
+    solve return FLUTAS_ENONFINITE (four-pass pipeline: from P1's loads;
+    five-pass line pipeline: a flag pass); the next finite solve succeeds."""
+    ng, lengths, alpha = (64, 32, 24), (2 * np.pi, np.pi, 1.0), 1.5
+    _, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], alpha, lengths[2])
+    dli = tuple(n / l for n, l in zip(ng, lengths))
+    monkeypatch.setenv("FLUTAS_UNFUSED", unfused)
+    plan = api.PoissonPlan(ng, dli, dzci, dzfi, "PPPPNN")
+    f = _rhs(ng, dzf, True, seed=8)
+    fbad = f.copy(order="F")
+    fbad[17, 5, 9] = bad
+    d = api.DeviceArray.from_host(fbad)
+    with pytest.raises(api.FlutasError) as e:
+        plan.solve(d)
+    assert e.value.code == api.FLUTAS_ENONFINITE
+    d = api.DeviceArray.from_host(f)
+    plan.solve(d)
+    assert np.all(np.isfinite(d.to_host()))
+    plan.destroy()

@@ -6,6 +6,11 @@
   twophase_256x256x512_periodic  triply periodic two-phase step
       (pres_tw_src with the constant-coefficient splitting, fillps, solve,
       correc) against the oracle, plain FP64 and FP32 + refinement.
+  channel_512_stretched          the headline 512^3 stretched channel
+      (PP/PP/NN, tanh stretching 1.5): FP64 and FP32 + refinement against
+      the oracle's solve of the same right-hand side.
+  batch_problems()               the seeded 64-problem uneven batch: every
+      problem against the oracle.
   neumann_1024x1024x512          NN/NN/NN (4 GiB per FP64 field): too
       large for the oracle in a test, checked through size-independent
       properties: the relative residual of the FP64 solve, the refined
@@ -94,12 +99,14 @@ def test_twophase_256x256x512_step(api, oracle, precision):
     pold = np.asfortranarray(g.standard_normal((nx + 2, ny + 2, nz + 2)))
     rho = wl.sphere_density(case)
     rho0 = float(rho.min())
-    dt = 1e-3
+    # dt != dto, so the extrapolated pressure (1 + dt/dto) p - (dt/dto) pold
+    # of the splitting differs from p and pold
+    dt, dto = 1e-3, 1.25e-3
     print(f"{case.name} {case.ng} precision {precision}", flush=True)
     D = api.DeviceArray.from_host
     ud, vd, wd, pd, poldd, rhod = D(u), D(v), D(w), D(p), D(pold), D(rho)
     dzcid, dzfid = D(dzci), D(dzfi)
-    api.pres_tw_src(nx, ny, nz, dli[0], dli[1], dli[2], 1, 1, dzcid, 1.0 / rho0, dt, dt,
+    api.pres_tw_src(nx, ny, nz, dli[0], dli[1], dli[2], 1, 1, dzcid, 1.0 / rho0, dt, dto,
                     pd, poldd, rhod, ud, vd, wd)
     # velocity halos after the source term, as bounduvw sets them
     uh, vh, wh = _halos(case, ud.to_host(), vd.to_host(), wd.to_host())
@@ -113,7 +120,9 @@ def test_twophase_256x256x512_step(api, oracle, precision):
     relres = plan.residual(fd, sol)
     # oracle
     uo, vo, wo = (a.copy(order="F") for a in (u, v, w))
-    oracle.pres_tw_src(case.ng, dli, 1, 1, dzci, 1.0 / rho0, dt, dt, p, pold, rho, uo, vo, wo)
+    oracle.pres_tw_src(case.ng, dli, 1, 1, dzci, 1.0 / rho0, dt, dto, p, pold, rho, uo, vo, wo)
+    for g, o in ((ud, uo), (vd, vo), (wd, wo)):
+        np.testing.assert_array_equal(interior(g.to_host()), interior(o))
     _halos(case, uo, vo, wo)
     fo = np.zeros_like(f)
     oracle.fillps(case.ng, 1, 1, dli, dzfi, 1.0 / dt, rho0, uo, vo, wo, fo)
@@ -152,3 +161,76 @@ def test_neumann_1024x1024x512_solve(api):
     # both solve the same singular system with the same pinned gauge
     a, b = p64.to_host(), p32.to_host()
     assert relmax(gauge_free(interior(b)), gauge_free(interior(a))) <= 1e-8
+
+
+def test_channel_512_stretched_solve(api, oracle):
+    """The headline workload: FP64 <= 1e-12 relative max-norm against the
+    oracle (same pinned gauge) with a relative residual <= 1e-12; FP32 +
+    FP64 refinement to a 1e-10 residual, <= 1e-8 gauge-free against the
+    oracle (src/solver_cpu.f90:20-115)."""
+    from flutas_amd import workloads as wl
+    case = wl.WORKLOADS["channel_512_stretched"]
+    dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
+    print(f"{case.name} {case.ng} {case.cbc} alpha={case.alpha}", flush=True)
+    f = wl.rhs_normal(case, dzf)
+    fd = api.DeviceArray.from_host(f)
+    pd = api.DeviceArray(fd.shape)
+    plan = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc)
+    plan.solve_into(fd, pd)
+    relres = plan.residual(fd, pd)
+    got64 = interior(pd.to_host())
+    plan.destroy()
+    plan32 = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc,
+                             precision=api.FP32_REFINE, tol=1e-10)
+    plan32.solve_into(fd, pd)
+    iters, rel32 = plan32.stats()
+    check32 = plan32.residual(fd, pd)
+    got32 = interior(pd.to_host())
+    plan32.destroy()
+    del fd, pd
+    print(f"gpu done: relres64 {relres:.2e}, refined {iters} iterations relres {rel32:.2e}",
+          flush=True)
+    s = oracle.Solver(case.ng, case.dli, dzci, dzfi, case.cbc)
+    s.solve(f)
+    ref = interior(f)
+    err64 = relmax(got64, ref)
+    err32 = relmax(gauge_free(got32), gauge_free(ref))
+    print(f"oracle: fp64 err {err64:.2e}, refined err {err32:.2e}", flush=True)
+    assert relres <= 1e-12, relres
+    assert err64 <= 1e-12, err64
+    assert rel32 <= 1e-10 and check32 <= 1e-10 and 1 <= iters <= 4, (iters, rel32, check32)
+    assert err32 <= 1e-8, err32
+
+
+def test_batch_problems_against_oracle(api, oracle):
+    """The seeded uneven batch (64 problems, nx, ny in {64..256}, nz in
+    {64, 128, 256}, PP/PP/NN, NN/NN/NN and PP/PP/DD, half of the z grids
+    stretched): every problem of the batched solve against the oracle,
+    <= 1e-12 relative max-norm with the same gauge pin."""
+    from flutas_amd import workloads as wl
+    cases = wl.batch_problems()
+    assert len(cases) == 64
+    problems, fs, ps, rhs = [], [], [], []
+    for case in cases:
+        dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
+        f = wl.rhs_normal(case, dzf)
+        problems.append((case.ng, case.dli, dzci, dzfi, case.cbc))
+        rhs.append(f)
+        fs.append(api.DeviceArray.from_host(f))
+        p = api.DeviceArray(f.shape)
+        p.zero()
+        ps.append(p)
+    print(f"batch of {len(cases)} problems, {sum(c.npoints for c in cases)} points", flush=True)
+    batch = api.BatchPlan(problems)
+    batch.set_fields(fs, ps)
+    batch.solve()
+    worst = 0.0
+    for q, (case, (ng, dli, dzci, dzfi, cbc), f, p) in enumerate(zip(cases, problems, rhs, ps)):
+        s = oracle.Solver(ng, dli, dzci, dzfi, cbc)
+        ref = f.copy(order="F")
+        s.solve(ref)
+        err = relmax(interior(p.to_host()), interior(ref))
+        worst = max(worst, err)
+        assert err <= 1e-12, (q, case.ng, case.cbc, case.alpha, err)
+    print(f"worst relative difference vs oracle: {worst:.2e}", flush=True)
+    batch.destroy()

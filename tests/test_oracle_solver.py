@@ -299,3 +299,114 @@ def test_pres_tw_src_reduces_to_correc(oracle):
     oracle.correc(ng, 1, 1, dli, dzci, 0.3, 4.0, pold, ub, vb, wb)
     for x, y in ((ua, ub), (va, vb), (wa, wb)):
         assert relmax(x, y) < 1e-14
+
+
+def _checkerboard_rho(shape):
+    """rho = 1 or 3 in a checkerboard: every face between two neighbouring
+    cells has one of each, so the arithmetic face density is exactly 2 (a
+    harmonic mean would give 1.5)."""
+    i, j, k = np.indices(shape)
+    return np.asfortranarray(1.0 + 2.0 * ((i + j + k) % 2))
+
+
+@pytest.mark.parametrize("const_coeff", [True, False])
+def test_pres_tw_src_linear_in_time(oracle, const_coeff):
+    """src/source.f90:268-269, :291-296 on a pressure linear in time,
+    p(t) = A + B t, with pold = p(t - dto), p = p(t) and dt != dto: the
+    extrapolated pressure f1 p - f2 pold is p(t + dt), so the increment is
+    dt [-grad p(t)/rho0 - (1/rho_f - 1/rho0) grad p(t + dt)] with the face
+    gradients of A + B t taken directly (constant-coefficient branch), or
+    -dt grad p(t)/rho_f (variable-coefficient branch).  rho_f = 2 exactly
+    (checkerboard density); a swapped dt/dto ratio fails."""
+    ng = (6, 7, 8)
+    rng = np.random.default_rng(21)
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, ng[2], 1.5, 1.0)
+    dli = (6.0, 3.5, 8.0)
+    A, B = halo_field(ng, 1, rng), halo_field(ng, 1, rng)
+    t, dt, dto, rho0 = 0.7, 0.3, 0.2, 1.0
+    p = np.asfortranarray(A + B * t)
+    pold = np.asfortranarray(A + B * (t - dto))
+    pnext = A + B * (t + dt)
+    rho = _checkerboard_rho(p.shape)
+    u0, v0, w0 = (halo_field(ng, 1, rng) for _ in range(3))
+    u, v, w = (a.copy(order="F") for a in (u0, v0, w0))
+    oracle.pres_tw_src(ng, dli, 1, 1, dzci, 1 / rho0, dt, dto, p, pold, rho, u, v, w,
+                       const_coeff=const_coeff)
+    I = slice(1, -1)
+    nz = ng[2]
+    metric = (dli[0], dli[1], dzci[1:nz + 1][None, None, :])
+    for d, (new, old) in enumerate(((u, u0), (v, v0), (w, w0))):
+        nb = [I, I, I]
+        nb[d] = slice(2, None)
+
+        def grad(f):
+            return (f[tuple(nb)] - f[I, I, I]) * metric[d]
+        if const_coeff:
+            want = dt * (-grad(p) / rho0 - (1 / 2.0 - 1 / rho0) * grad(pnext))
+        else:
+            want = -dt * grad(p) / 2.0
+        got = new[I, I, I] - old[I, I, I]
+        assert relmax(got, want) < 1e-12, d
+    if const_coeff:
+        # the same call with the ratio inverted misses p(t + dt)
+        us = u0.copy(order="F")
+        oracle.pres_tw_src(ng, dli, 1, 1, dzci, 1 / rho0, dto, dt, p, pold, rho, us,
+                           v0.copy(order="F"), w0.copy(order="F"))
+        want = dto * (-(p[2:, I, I] - p[I, I, I]) * dli[0] / rho0
+                      - (1 / 2.0 - 1 / rho0) * (pnext[2:, I, I] - pnext[I, I, I]) * dli[0])
+        assert relmax(us[I, I, I] - u0[I, I, I], want) > 1e-3
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+def test_fillps_linear_velocity(oracle, alpha):
+    """src/fillps.f90:50-57: u = a x_face, v = b y_face, w = c z_face (the
+    stretched faces of initgrid) have the divergence a + b + c in every
+    cell, so fillps gives rho0 (a + b + c) dti exactly (to rounding); a
+    dropped rho0 or dti fails."""
+    ng = (5, 6, 9)
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, zc, zf = z_metrics(oracle, nz, alpha, 1.3)
+    dli = (4.0, 2.5, nz / 1.3)
+    ca, cb, cc = 0.7, -1.9, 2.3
+    # u(i) sits on the face x = i dx, i = 0..nx+1 (halo 1)
+    xf = np.arange(0, nx + 2) / dli[0]
+    yf = np.arange(0, ny + 2) / dli[1]
+    shape = (nx + 2, ny + 2, nz + 2)
+    u = np.asfortranarray(np.broadcast_to(ca * xf[:, None, None], shape))
+    v = np.asfortranarray(np.broadcast_to(cb * yf[None, :, None], shape))
+    w = np.asfortranarray(np.broadcast_to(cc * zf[None, None, :], shape))
+    p = halo_field(ng, 1)
+    dti, rho0 = 1 / 0.04, 2.5
+    oracle.fillps(ng, 1, 1, dli, dzfi, dti, rho0, u, v, w, p)
+    want = rho0 * (ca + cb + cc) * dti
+    np.testing.assert_allclose(interior(p), want, rtol=1e-11)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.5])
+@pytest.mark.parametrize("const_coeff", [True, False])
+def test_correc_linear_pressure(oracle, alpha, const_coeff):
+    """src/correc.f90:57-68: p = a x_c + b y_c + c z_c has the face
+    gradient (a, b, c) everywhere, so correc lowers u, v, w by
+    dt (a, b, c)/rho0 (constant coefficients) or dt (a, b, c)/2
+    (checkerboard density, arithmetic face mean 2).  z_c is accumulated
+    from initgrid's dzc (stretched), so that its differences are the
+    spacings correc divides by, ghost layers included."""
+    ng = (6, 5, 8)
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, zc, zf = z_metrics(oracle, nz, alpha, 0.8)
+    dli = (3.0, 5.0, nz / 0.8)
+    ca, cb, cc = 1.1, -0.6, 0.35
+    xc = (np.arange(0, nx + 2) - 0.5) / dli[0]
+    yc = (np.arange(0, ny + 2) - 0.5) / dli[1]
+    zcl = np.concatenate([[0.0], np.cumsum(dzc[:nz + 1])])
+    p = np.asfortranarray(ca * xc[:, None, None] + cb * yc[None, :, None] +
+                          cc * zcl[None, None, :])
+    rng = np.random.default_rng(3)
+    u0, v0, w0 = (halo_field(ng, 1, rng) for _ in range(3))
+    u, v, w = (a.copy(order="F") for a in (u0, v0, w0))
+    dt, rho0 = 0.05, 4.0
+    rho = _checkerboard_rho(p.shape)
+    oracle.correc(ng, 1, 1, dli, dzci, dt, rho0, p, u, v, w, rho=rho, const_coeff=const_coeff)
+    scale = rho0 if const_coeff else 2.0
+    for new, old, g in ((u, u0, ca), (v, v0, cb), (w, w0, cc)):
+        np.testing.assert_allclose(interior(old) - interior(new), dt * g / scale, rtol=1e-10)

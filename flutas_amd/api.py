@@ -477,8 +477,8 @@ def load_scalar(io, filename, time=0.0, istep=0, dto=0.0):
     return t.value, it.value, d.value
 
 
-R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
-            "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
+R2R_KIND = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
+            "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 
 
 def fft_lines(kind, data, n, howmany, stride=1, dist=None):

@@ -81,14 +81,14 @@ __global__ void __launch_bounds__(kBatchThreads)
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_yz(const BatchProblem &q, int i, double *dyn) {
   yz_fwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
-      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.ky_f, q.lamx, q.lamy, q.a, q.b, q.c,
+      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.nz, q.ky_f, q.lamx, q.lamy, q.a, q.b, q.c,
       q.pin, q.twy, q.twhy,
       q.twqy, dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }
 template <int NC, int GP, bool PER>
 __device__ __forceinline__ void batch_zy(const BatchProblem &q, int i, double *dyn) {
   zy_bwd_slab<double, NC, GP, batch_slab_threads<NC>(), PER>(
-      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.ky_b,
+      q.W, plane_major(q.nx, q.ny), q.ck, q.nx, q.nz, q.nz, q.ky_b,
       q.lamx, q.lamy, q.a, q.b, q.c, q.twy, q.twhy, q.twqy,
       dyn, dyn + 2 * GP * line_stride<double, 2 * NC>(), i);
 }

@@ -341,7 +341,81 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// REDFT00 / RODFT00 (DCT-I / DST-I, the face-centred NN and DD kinds of
+// src/fft.f90:265-289) through the real DFT of the symmetric extension of
+// each line of n values, length m = 2(n-1) (even) or 2(n+1) (odd):
+//   even  e_j = x_j (j < n), e_{m-j} = x_j (0 < j < n-1):
+//         E_k real, y_k = E_k = x_0 + (-1)^k x_{n-1}
+//                             + 2 sum_{0<j<n-1} x_j cos(pi j k/(n-1));
+//   odd   o_0 = o_{n+1} = 0, o_{j+1} = x_j, o_{m-1-j} = -x_j:
+//         O_k = -i 2 sum_j x_j sin(pi (j+1) k/(n+1)), y_k = -Im O_{k+1}.
+// The plan pl is that of length m (N = m/2 complex points); g.n = n.
+template <typename T, typename Tin, typename Tout>
+__global__ void __launch_bounds__(256)
+    k_lines_ext(const Tin *__restrict__ in, Tout *__restrict__ out, LineGeom g,
+                LinePlan<T> pl, int kind, T scale, int lpb) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n = g.n, m = pl.n, N = pl.N;
+  T *sA = reinterpret_cast<T *>(smem_raw);
+  T *sB = sA + (size_t)lpb * m;
+  const int l1_0 = blockIdx.x * lpb;
+  const int l2 = blockIdx.y;
+  const int nl = min(lpb, g.L1 - l1_0);
+  const bool odd = (kind == K_RODFT00);
+  if (odd) {
+    for (int t = threadIdx.x; t < nl; t += blockDim.x) {
+      sA[t * m] = T(0);
+      sA[t * m + n + 1] = T(0);
+    }
+  }
+  // 1. load x and place it (and its mirror image) in the extension
+  for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+    int l, e;
+    if (g.ise == 1) { l = t / n; e = t - l * n; }
+    else { e = t / nl; l = t - e * nl; }
+    const T x = (T)in[(long long)(l1_0 + l) * g.is1 + (long long)l2 * g.is2 +
+                      (long long)e * g.ise];
+    T *E = sA + l * m;
+    if (odd) {
+      E[e + 1] = x;
+      E[m - 1 - e] = -x;
+    } else {
+      E[e] = x;
+      if (e > 0 && e < n - 1) E[m - e] = x;
+    }
+  }
+  __syncthreads();
+  // 2. complex FFT of the packed extension (N points)
+  T *src = sA, *dst = sB;
+  fft_stages<T, -1>(src, dst, pl, nl, m);
+  // 3. real split X_k = (Z_k + conj Z_{N-k})/2 - i e^{-2 pi i k/m}
+  //    (Z_k - conj Z_{N-k})/2 (Z_N := Z_0) for the k the kind needs
+  for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+    const int l = t / n, e = t - (t / n) * n;
+    const int k = odd ? e + 1 : e;  // k <= N
+    const cpx<T> *Z = reinterpret_cast<const cpx<T> *>(src + l * m);
+    const cpx<T> Zk = Z[(k == N) ? 0 : k];
+    const cpx<T> Zm = Z[(k == 0) ? 0 : N - k];
+    const cpx<T> A = cadd(Zk, cconj(Zm)), B = csub(Zk, cconj(Zm));
+    const cpx<T> wB = cmul(cpx<T>{pl.twh[2 * k], pl.twh[2 * k + 1]}, B);
+    dst[l * m + e] = odd ? -(T(0.5) * A.y - T(0.5) * wB.x) : (T(0.5) * A.x + T(0.5) * wB.y);
+  }
+  __syncthreads();
+  // 4. store
+  for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+    int l, e;
+    if (g.ose == 1) { l = t / n; e = t - l * n; }
+    else { e = t / nl; l = t - e * nl; }
+    out[(long long)(l1_0 + l) * g.os1 + (long long)l2 * g.os2 + (long long)e * g.ose] =
+        (Tout)(dst[l * m + e] * scale);
+  }
+}
+
 // ---- host side -------------------------------------------------------------
+
+// Largest prime factor of N a plan accepts (direct DFT of that size per
+// butterfly, O(p^2) operations).
+constexpr int kMaxPrime = 1021;
 
 static int factorize(int N, int *radix) {
   int nst = 0;
@@ -356,7 +430,7 @@ static int factorize(int N, int *radix) {
   }
   for (int p = 7; m > 1; p += 2) {
     while (m % p == 0) {
-      if (nst >= kMaxStages || p > 64) return -1;
+      if (nst >= kMaxStages || p > kMaxPrime) return -1;
       radix[nst++] = p;
       m /= p;
     }
@@ -385,7 +459,7 @@ int LineTables::build(int n_) {
   n = n_;
   N = n / 2;
   nst = (N == 1) ? 0 : factorize(N, radix);
-  if (nst < 0) return fail(FLUTAS_EUNSUPPORTED, "n/2 has a prime factor > 64");
+  if (nst < 0) return fail(FLUTAS_EUNSUPPORTED, "n/2 has a prime factor > 1021");
   const long double pi = 3.14159265358979323846264338327950288L;
   std::vector<double> tw(2 * N), twh(2 * (N + 1)), twq(2 * (n + 1)), twe(2 * N);
   for (int m = 0; m < N; ++m) {
@@ -437,9 +511,10 @@ void LineTables::release() {
 template <typename T, typename Tin, typename Tout>
 int launch_lines(const LineTables &tab, int kind, const Tin *in, Tout *out,
                  const LineGeom &g, T scale, hipStream_t s) {
-  if (g.n != tab.n) return fail(FLUTAS_EINVAL, "line length does not match plan");
+  const int m = dft_length(kind, g.n);
+  if (m != tab.n) return fail(FLUTAS_EINVAL, "line length does not match plan");
   // lines per block: as many as fit 64 KiB of LDS (two buffers), <= 16
-  const size_t per_line = 2 * (size_t)g.n * sizeof(T);
+  const size_t per_line = 2 * (size_t)m * sizeof(T);
   int lpb = (int)std::min<size_t>(16, (64 * 1024) / per_line);
   if (lpb < 1) return fail(FLUTAS_EUNSUPPORTED, "line too long for one workgroup");
   lpb = std::min(lpb, g.L1);
@@ -447,8 +522,12 @@ int launch_lines(const LineTables &tab, int kind, const Tin *in, Tout *out,
   const size_t shmem = per_line * lpb;
   LinePlan<T> pl;
   if constexpr (sizeof(T) == 8) pl = tab.dplan(); else pl = tab.fplan();
-  hipLaunchKernelGGL((k_lines<T, Tin, Tout>), grid, dim3(256), shmem, s, in,
-                     out, g, pl, kind, scale, lpb);
+  if (m != g.n)
+    hipLaunchKernelGGL((k_lines_ext<T, Tin, Tout>), grid, dim3(256), shmem, s, in,
+                       out, g, pl, kind, scale, lpb);
+  else
+    hipLaunchKernelGGL((k_lines<T, Tin, Tout>), grid, dim3(256), shmem, s, in,
+                       out, g, pl, kind, scale, lpb);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }

@@ -34,29 +34,29 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
 
 template <typename T, int NC, int GP, int NT, bool PER>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
-    k_yz_fwd(T *__restrict__ Wt, WStride ws, double *__restrict__ ck, int nx, int nz, int kind,
-             const double *__restrict__ lamx, const double *__restrict__ lamy,
+    k_yz_fwd(T *__restrict__ Wt, WStride ws, double *__restrict__ ck, int nx, int nz, int nzs,
+             int kind, const double *__restrict__ lamx, const double *__restrict__ lamy,
              const double *__restrict__ a, const double *__restrict__ b,
              const double *__restrict__ c, int pin, const T *__restrict__ tw,
              const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  yz_fwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, kind, lamx, lamy, a, b, c, pin, tw, twh,
-                                  twq, sm, stw, blockIdx.x);
+  yz_fwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, tw,
+                                  twh, twq, sm, stw, blockIdx.x);
 }
 
 template <typename T, int NC, int GP, int NT, bool PER>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_zy_bwd(T *__restrict__ Wt, WStride ws, const double *__restrict__ ck, int nx, int nz,
-             int kind,
+             int nzs, int kind,
              const double *__restrict__ lamx, const double *__restrict__ lamy,
              const double *__restrict__ a, const double *__restrict__ b,
              const double *__restrict__ c, const T *__restrict__ tw,
              const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  zy_bwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, kind, lamx, lamy, a, b, c, tw, twh, twq,
-                                  sm, stw, blockIdx.x);
+  zy_bwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, tw, twh,
+                                  twq, sm, stw, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host ---
@@ -204,7 +204,7 @@ int fused_group_planes(int ny, int elem_bytes) {
 
 template <typename T>
 int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
-                  double *ck, int nx, int nz, const double *lamx,
+                  double *ck, int nx, int nz, int nzs, const double *lamx,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int pin, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
@@ -214,10 +214,12 @@ int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
-                         ck, nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh,
+                         pl.twq);
     else
       hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
-                         ck, nx, nz, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh,
+                         pl.twq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -225,7 +227,7 @@ int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
 
 template <typename T>
 int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
-                  const double *ck, int nx, int nz, const double *lamx,
+                  const double *ck, int nx, int nz, int nzs, const double *lamx,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
@@ -235,10 +237,10 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
-                         ck, nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
     else
       hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
-                         ck, nx, nz, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -249,9 +251,9 @@ template int launch_x_fwd<float, double>(const LineTables &, int, const double *
 template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_bwd<double, double>(const LineTables &, int, const double *, WStride, double *, long long, long long, int, int, double, int, hipStream_t);
 template int launch_x_bwd<float, double>(const LineTables &, int, const float *, WStride, double *, long long, long long, int, int, float, int, hipStream_t);
-template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
-template int launch_yz_fwd<float>(const LineTables &, int, float *, WStride, double *, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
-template int launch_zy_bwd<double>(const LineTables &, int, double *, WStride, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
-template int launch_zy_bwd<float>(const LineTables &, int, float *, WStride, const double *, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
+template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_yz_fwd<float>(const LineTables &, int, float *, WStride, double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_zy_bwd<double>(const LineTables &, int, double *, WStride, const double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
+template int launch_zy_bwd<float>(const LineTables &, int, float *, WStride, const double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
 
 }  // namespace flutas

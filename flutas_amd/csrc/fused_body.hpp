@@ -336,7 +336,7 @@ __host__ __device__ constexpr int checkpoints(int nz) {
 template <typename T, int NC, int GP, int NT, bool PER>
 __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
                                             double *__restrict__ ck,
-                                            int nx, int nz, int kind,
+                                            int nx, int nz, int nzs, int kind,
                                             const double *__restrict__ lamx,
                                             const double *__restrict__ lamy,
                                             const double *__restrict__ a,
@@ -450,11 +450,14 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
         const double y = (double)Y[l * LS + j];
         double pv;
         if constexpr (!PER) {
-          if (k == 0) {
+          if (k >= nzs) {
+            // a boundary plane outside the z system passes through
+            pv = y;
+          } else if (k == 0) {
             const double z = 1.0 / (bk + lam[u]);
             dprev[u] = cc * z;
             pv = y * z;
-          } else if (k < nz - 1) {
+          } else if (k < nzs - 1) {
             const double bb = bk + lam[u];
             const double z = 1.0 / (bb - ak * dprev[u]);
             dprev[u] = cc * z;
@@ -520,7 +523,7 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
 template <typename T, int NC, int GP, int NT, bool PER>
 __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
                                             const double *__restrict__ ck,
-                                            int nx, int nz, int kind,
+                                            int nx, int nz, int nzs, int kind,
                                             const double *__restrict__ lamx,
                                             const double *__restrict__ lamy,
                                             const double *__restrict__ a,
@@ -612,7 +615,7 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
 #pragma unroll
         for (int l = 0; l < GP; ++l) {
           const int k = k0 + l;
-          if (l < np && k < nz - 1) {
+          if (l < np && k < nzs - 1) {
             if (k == 0) dp = c[0] * (1.0 / (b[0] + lam[u]));
             else dp = c[k] * (1.0 / ((b[k] + lam[u]) - a[k] * dp));
           }
@@ -623,7 +626,8 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
         for (int l = GP - 1; l >= 0; --l) {
           if (l >= np) continue;
           const int k = k0 + l;
-          const double p = (k == nz - 1) ? (double)pv[l][u] : (double)pv[l][u] - d[l] * pn;
+          // the last row of the system and the planes past it keep p'
+          const double p = (k >= nzs - 1) ? (double)pv[l][u] : (double)pv[l][u] - d[l] * pn;
           pn = p;
           pv[l][u] = (T)p;
         }

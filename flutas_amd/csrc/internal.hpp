@@ -38,9 +38,15 @@ int check_device();  // EDEVICE unless a gfx950 device is current
 // Transform kinds (FFTW numbering, src/fftw.f90:38-61).
 enum Kind {
   K_R2HC = 0, K_HC2R = 1,
-  K_REDFT01 = 4, K_REDFT10 = 5, K_REDFT11 = 6,
-  K_RODFT01 = 8, K_RODFT10 = 9, K_RODFT11 = 10
+  K_REDFT00 = 3, K_REDFT01 = 4, K_REDFT10 = 5, K_REDFT11 = 6,
+  K_RODFT00 = 7, K_RODFT01 = 8, K_RODFT10 = 9, K_RODFT11 = 10
 };
+// Length of the real DFT that evaluates a kind on lines of n values: the
+// symmetric extension 2(n-1) for REDFT00 and 2(n+1) for RODFT00, n
+// otherwise.
+inline int dft_length(int kind, int n) {
+  return kind == K_REDFT00 ? 2 * (n - 1) : (kind == K_RODFT00 ? 2 * (n + 1) : n);
+}
 
 constexpr int kMaxStages = 16;
 
@@ -151,12 +157,12 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
                  T scale, int accumulate, hipStream_t s);
 template <typename T>
 int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
-                  double *ck, int nx, int nz, const double *lamx,
+                  double *ck, int nx, int nz, int nzs, const double *lamx,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int pin, int periodic, hipStream_t s);
 template <typename T>
 int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
-                  const double *ck, int nx, int nz, const double *lamx,
+                  const double *ck, int nx, int nz, int nzs, const double *lamx,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int periodic, hipStream_t s);
 

@@ -47,7 +47,8 @@ int check_device() {
 // pi as the reference defines it: acos(-1._rp) (param.f90:99)
 static double ref_pi() { return std::acos(-1.0); }
 
-// eigenvalues, 'c' branch (src/initsolver.f90:122-186, CPU ordering)
+// eigenvalues, 'c' branch (src/initsolver.f90:122-186, CPU ordering); the
+// face-centred PP, ND and DN pairs use the same formulas (:136-140, :180-183)
 static void eigen_c(int n, char b0, char b1, std::vector<double> &lam) {
   const double pi = ref_pi();
   lam.assign(n, 0.0);
@@ -61,43 +62,71 @@ static void eigen_c(int n, char b0, char b1, std::vector<double> &lam) {
   }
 }
 
-// tridmatrix, 'c' (src/initsolver.f90:188-246); zci[k] == dzci(k)
-static void tridmatrix_c(char b0, char b1, int n, const double *zci,
-                         const double *zfi, std::vector<double> &a,
-                         std::vector<double> &b, std::vector<double> &c) {
+// tridmatrix (src/initsolver.f90:188-246); zci[k] == dzci(k).  Cell-centred
+// ('c') unknowns couple through dzci(k-1), dzci(k); face-centred ('f') ones
+// through dzfi(k), dzfi(k+1), and only an N face modifies the diagonal (a D
+// face value is no unknown).
+static void tridmatrix(char b0, char b1, int n, const double *zci, const double *zfi,
+                       char cf, std::vector<double> &a, std::vector<double> &b,
+                       std::vector<double> &c) {
   a.resize(n); b.resize(n); c.resize(n);
   for (int k = 1; k <= n; ++k) {
-    a[k - 1] = zfi[k] * zci[k - 1];
-    c[k - 1] = zfi[k] * zci[k];
+    if (cf == 'c') {
+      a[k - 1] = zfi[k] * zci[k - 1];
+      c[k - 1] = zfi[k] * zci[k];
+    } else {
+      a[k - 1] = zfi[k] * zci[k];
+      c[k - 1] = zfi[k + 1] * zci[k];
+    }
   }
   for (int k = 0; k < n; ++k) b[k] = -(a[k] + c[k]);
   auto fac = [](char t) { return t == 'P' ? 0.0 : (t == 'D' ? -1.0 : 1.0); };
-  b[0] = b[0] + fac(b0) * a[0];
-  b[n - 1] = b[n - 1] + fac(b1) * c[n - 1];
+  if (cf == 'c') {
+    b[0] = b[0] + fac(b0) * a[0];
+    b[n - 1] = b[n - 1] + fac(b1) * c[n - 1];
+  } else {
+    if (b0 == 'N') b[0] = b[0] + fac(b0) * a[0];
+    if (b1 == 'N') b[n - 1] = b[n - 1] + fac(b1) * c[n - 1];
+  }
 }
 
-// bc_rhs, 'c' (src/initsolver.f90:248-298)
-static void bc_rhs_c(const char *cbc, const double *bc, const double dlc[2],
-                     const double dlf[2], long plane, std::vector<double> &rhs) {
+// bc_rhs (src/initsolver.f90:248-298)
+static void bc_rhs(const char *cbc, const double *bc, const double dlc[2],
+                   const double dlf[2], char cf, long plane, std::vector<double> &rhs) {
   rhs.assign(2 * plane, 0.0);
   for (int ib = 0; ib < 2; ++ib) {
     double factor = 0.0;
     const double sgn = (ib == 0) ? 1.0 : -1.0;
-    if (cbc[ib] == 'D') factor = -2.0 * bc[ib];
-    else if (cbc[ib] == 'N') factor = sgn * dlc[ib] * bc[ib];
+    if (cbc[ib] == 'D') factor = (cf == 'c') ? -2.0 * bc[ib] : -bc[ib];
+    else if (cbc[ib] == 'N') factor = (cf == 'c') ? sgn * dlc[ib] * bc[ib] : sgn * dlf[ib] * bc[ib];
     const double v = factor / dlc[ib] / dlf[ib];
     std::fill(rhs.begin() + ib * plane, rhs.begin() + (ib + 1) * plane, v);
   }
 }
 
-// find_fft, 'c' (src/fft.f90:242-264)
-static int find_fft_c(char b0, char b1, int &kf, int &kb, double &norm0) {
-  if (b0 == 'P' && b1 == 'P') { kf = K_R2HC; kb = K_HC2R; norm0 = 1; return 0; }
-  if (b0 == 'N' && b1 == 'N') { kf = K_REDFT10; kb = K_REDFT01; norm0 = 2; return 0; }
-  if (b0 == 'D' && b1 == 'D') { kf = K_RODFT10; kb = K_RODFT01; norm0 = 2; return 0; }
-  if (b0 == 'N' && b1 == 'D') { kf = K_REDFT11; kb = K_REDFT11; norm0 = 2; return 0; }
-  if (b0 == 'D' && b1 == 'N') { kf = K_RODFT11; kb = K_RODFT11; norm0 = 2; return 0; }
-  return -1;
+// find_fft (src/fft.f90:233-291) for the kinds built here; -1 for an
+// unknown pair, -2 for the face-centred NN and DD pairs (REDFT00/RODFT00,
+// whose eigenvalues in src/initsolver.f90:164-178 do not match them).
+static int find_fft(char b0, char b1, char cf, int &kf, int &kb, double &norm0) {
+  const bool P = b0 == 'P' && b1 == 'P', NN = b0 == 'N' && b1 == 'N',
+             DD = b0 == 'D' && b1 == 'D', ND = b0 == 'N' && b1 == 'D',
+             DN = b0 == 'D' && b1 == 'N';
+  if (P) { kf = K_R2HC; kb = K_HC2R; norm0 = 1; return 0; }
+  if (cf == 'c') {
+    if (NN) { kf = K_REDFT10; kb = K_REDFT01; norm0 = 2; return 0; }
+    if (DD) { kf = K_RODFT10; kb = K_RODFT01; norm0 = 2; return 0; }
+    if (ND) { kf = K_REDFT11; kb = K_REDFT11; norm0 = 2; return 0; }
+    if (DN) { kf = K_RODFT11; kb = K_RODFT11; norm0 = 2; return 0; }
+    return -1;
+  }
+  if (ND) { kf = K_REDFT10; kb = K_REDFT01; norm0 = 2; return 0; }
+  if (DN) { kf = K_RODFT01; kb = K_RODFT10; norm0 = 2; return 0; }
+  return (NN || DD) ? -2 : -1;
+}
+
+// forward r2r kinds whose tile transform (post_fwd) the fused P1/P2 run
+static bool fused_forward_kind(int k) {
+  return k == K_R2HC || k == K_REDFT10 || k == K_RODFT10;
 }
 
 }  // namespace flutas
@@ -106,6 +135,10 @@ using namespace flutas;
 
 struct flutas_psolve {
   int nx, ny, nz;
+  // planes of the z system: nz, or nz - 1 for a face-centred z grid with a
+  // D top face, whose last plane is the boundary (src/solver_cpu.f90:71-72)
+  int nzs;
+  char cf[3];  // c_or_f
   char cbc[6];
   double bc[6];
   double dli[3];
@@ -241,8 +274,10 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
   if (per)
     return launch_thomas_periodic<T>(W, h->D, h->P2, h->d_lamx, h->d_lamy, h->d_a,
                                      h->d_b, h->d_c, h->nx, h->ny, h->nz, pin, s);
+  // a face-centred z grid with a D top face solves nz - 1 planes; the last
+  // one is left as it is (src/solver_cpu.f90:71-77)
   return launch_thomas<T>(W, h->D, h->d_lamx, h->d_lamy, h->d_a, h->d_b, h->d_c,
-                          h->nx, h->ny, h->nz, pin, s);
+                          h->nx, h->ny, h->nzs, pin, s);
 }
 
 // One spectral solve in storage/arithmetic T: the right-hand side `in`
@@ -266,11 +301,11 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
     FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ws, ny, nz, absmax_part,
                                  nf_flag, h->seq, s)));
     FL_TRY(mark(h, "P1_x_fwd", s));
-    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, nx, nz, h->d_lamx, h->d_lamy,
-                            h->d_a, h->d_b, h->d_c, pin, per, s));
+    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
+                            h->d_lamy, h->d_a, h->d_b, h->d_c, pin, per, s));
     FL_TRY(mark(h, "P2_yz_fwd", s));
-    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, nx, nz, h->d_lamx, h->d_lamy,
-                            h->d_a, h->d_b, h->d_c, per, s));
+    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
+                            h->d_lamy, h->d_a, h->d_b, h->d_c, per, s));
     FL_TRY(mark(h, "P3_zy_bwd", s));
     FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, ws, out, q.hx, q.hxy, ny, nz,
                                     (T)h->normfft, accumulate, s)));
@@ -455,7 +490,7 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
   *out = nullptr;
   for (int d = 0; d < 3; ++d) {
     if (ng[d] < 2 || (ng[d] & 1)) return fail(FLUTAS_EINVAL, "grid sizes must be even and >= 2 (src/sanity.f90:155)");
-    if (c_or_f[d] != 'c') return fail(FLUTAS_EUNSUPPORTED, "face-centred ('f') transforms are not built");
+    if (c_or_f[d] != 'c' && c_or_f[d] != 'f') return fail(FLUTAS_EINVAL, "c_or_f must be 'c' or 'f'");
   }
   for (int q = 0; q < 6; ++q)
     if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N') return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
@@ -467,21 +502,35 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
     return fail(FLUTAS_EINVAL, "precision must be 64 or 32");
   if (precision == FLUTAS_FP32_REFINE && !(tol > 0.0)) return fail(FLUTAS_EINVAL, "tol must be > 0");
   if (cbc[4] == 'P' && ng[2] < 4) return fail(FLUTAS_EINVAL, "periodic z needs nz >= 4");
+  const bool face = (c_or_f[0] == 'f' || c_or_f[1] == 'f' || c_or_f[2] == 'f');
+  if (face && precision == FLUTAS_FP32_REFINE)
+    return fail(FLUTAS_EUNSUPPORTED, "the refined mode's residual is the cell-centred operator; "
+                                     "face-centred ('f') plans are FP64 only");
+  const int qz = (c_or_f[2] == 'f' && cbc[5] == 'D') ? 1 : 0;
+  if (ng[2] - qz < 2) return fail(FLUTAS_EINVAL, "the z system needs at least two planes");
+  int kinds[4];
+  double n0x = 0.0, n0y = 0.0;
+  for (int d = 0; d < 2; ++d) {
+    const int r = find_fft(cbc[2 * d], cbc[2 * d + 1], c_or_f[d], kinds[2 * d], kinds[2 * d + 1],
+                           d == 0 ? n0x : n0y);
+    if (r == -2)
+      return fail(FLUTAS_EUNSUPPORTED, "face-centred NN/DD in x or y (REDFT00/RODFT00 solves) "
+                                       "is not built: src/initsolver.f90:164-178 pairs them "
+                                       "with eigenvalues of other transforms");
+    if (r) return fail(FLUTAS_EINVAL, "unknown boundary-condition pair in x or y");
+  }
   FL_TRY(check_device());
 
   auto *h = new flutas_psolve();
   h->nx = ng[0]; h->ny = ng[1]; h->nz = ng[2];
+  h->nzs = ng[2] - qz;
+  std::memcpy(h->cf, c_or_f, 3);
   std::memcpy(h->cbc, cbc, 6);
   std::memcpy(h->bc, bc, sizeof(double) * 6);
   std::memcpy(h->dli, dli, sizeof(double) * 3);
+  std::memcpy(h->kinds, kinds, sizeof(kinds));
   h->precision = precision;
   h->tol = tol;
-  double n0x, n0y;
-  if (find_fft_c(cbc[0], cbc[1], h->kinds[0], h->kinds[1], n0x) ||
-      find_fft_c(cbc[2], cbc[3], h->kinds[2], h->kinds[3], n0y)) {
-    delete h;
-    return fail(FLUTAS_EINVAL, "unknown boundary-condition pair in x or y");
-  }
   // normfft = 1/(norm(1)*(n+norm(2))) per direction (src/fft.f90:87,125,150)
   h->normfft = 1.0 / ((1.0 * n0x * (h->nx + 0.0)) * n0y * (h->ny + 0.0));
   eigen_c(h->nx, cbc[0], cbc[1], h->lamx);
@@ -489,15 +538,18 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
   eigen_c(h->ny, cbc[2], cbc[3], h->lamy);
   for (double &v : h->lamy) v = v * (dli[1] * dli[1]);
   const double *zci = dzci + (nh_d - 1), *zfi = dzfi + (nh_d - 1);
-  tridmatrix_c(cbc[4], cbc[5], h->nz, zci, zfi, h->a, h->b, h->c);
+  tridmatrix(cbc[4], cbc[5], h->nz, zci, zfi, c_or_f[2], h->a, h->b, h->c);
   {
     const double dlx[2] = {1.0 / dli[0], 1.0 / dli[0]};
     const double dly[2] = {1.0 / dli[1], 1.0 / dli[1]};
-    bc_rhs_c(cbc + 0, bc + 0, dlx, dlx, (long)h->ny * h->nz, h->rhsbx);
-    bc_rhs_c(cbc + 2, bc + 2, dly, dly, (long)h->nx * h->nz, h->rhsby);
-    const double dlc[2] = {1.0 / zci[0], 1.0 / zci[h->nz]};
+    bc_rhs(cbc + 0, bc + 0, dlx, dlx, c_or_f[0], (long)h->ny * h->nz, h->rhsbx);
+    bc_rhs(cbc + 2, bc + 2, dly, dly, c_or_f[1], (long)h->nx * h->nz, h->rhsby);
+    // the z spacings of the boundary rows: dzc(0), dzc(nz) cell-centred,
+    // dzc(1), dzc(nz-1) face-centred (src/initsolver.f90:109-113)
+    const bool fz = (c_or_f[2] == 'f');
+    const double dlc[2] = {1.0 / zci[fz ? 1 : 0], 1.0 / zci[fz ? h->nz - 1 : h->nz]};
     const double dlf[2] = {1.0 / zfi[1], 1.0 / zfi[h->nz]};
-    bc_rhs_c(cbc + 4, bc + 4, dlc, dlf, (long)h->nx * h->ny, h->rhsbz);
+    bc_rhs(cbc + 4, bc + 4, dlc, dlf, c_or_f[2], (long)h->nx * h->ny, h->rhsbz);
   }
   int rc = FLUTAS_OK;
   const size_t N = (size_t)h->nx * h->ny * h->nz;
@@ -525,10 +577,10 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
     }
     // the fused pipeline covers the compiled line lengths
     const char *env = std::getenv("FLUTAS_UNFUSED");
-    // (the tile transforms cover the R2HC, DCT-II/III and DST-II/III kinds)
-    bool quarter = false;
-    for (int kk = 0; kk < 4; ++kk) quarter |= (h->kinds[kk] == K_REDFT11 || h->kinds[kk] == K_RODFT11);
-    h->fused = fused_supported(h->nx) && fused_supported(h->ny) && h->nz >= 3 && !quarter &&
+    // (the tile transforms cover the R2HC, DCT-II/III and DST-II/III kinds
+    // run forward in P1/P2 and backward in P3/P4)
+    const bool tile_kinds = fused_forward_kind(h->kinds[0]) && fused_forward_kind(h->kinds[2]);
+    h->fused = fused_supported(h->nx) && fused_supported(h->ny) && h->nz >= 3 && tile_kinds &&
                !(env && env[0] == '1');
     if (h->fused) {
       const int gp = fused_group_planes(h->ny, precision == FLUTAS_FP64 ? 8 : 4);
@@ -679,10 +731,15 @@ int flutas_fft_guru(int kind, int n, long long stride, int howmany1, long long d
   if (!data || howmany1 < 1 || howmany2 < 1 || howmany2 > 65535 || stride < 1 || n < 2)
     return fail(FLUTAS_EINVAL, "bad fft arguments");
   if (kind != K_R2HC && kind != K_HC2R && kind != K_REDFT01 && kind != K_REDFT10 &&
-      kind != K_RODFT01 && kind != K_RODFT10 && kind != K_REDFT11 && kind != K_RODFT11)
-    return fail(FLUTAS_EUNSUPPORTED, "r2r kind not built");
+      kind != K_RODFT01 && kind != K_RODFT10 && kind != K_REDFT11 && kind != K_RODFT11 &&
+      kind != K_REDFT00 && kind != K_RODFT00)
+    return fail(FLUTAS_EINVAL, "unknown r2r kind");
+  // DCT-I / DST-I take any n >= 2 (their extension has even length), the
+  // other kinds an even n
+  if (kind != K_REDFT00 && kind != K_RODFT00 && (n & 1))
+    return fail(FLUTAS_EINVAL, "line length must be even for this kind");
   LineTables tab;
-  FL_TRY(tab.build(n));
+  FL_TRY(tab.build(dft_length(kind, n)));
   LineGeom g;
   g.n = n; g.L1 = howmany1; g.L2 = howmany2;
   g.is1 = dist1; g.is2 = dist2; g.ise = stride;
@@ -726,6 +783,8 @@ int flutas_psolve_stats(const flutas_psolve *h, int *iters, double *relres) {
 int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
                            double *relres) {
   if (!h || !f || !p || !relres) return fail(FLUTAS_EINVAL, "null pointer");
+  if (h->cf[0] != 'c' || h->cf[1] != 'c' || h->cf[2] != 'c')
+    return fail(FLUTAS_EUNSUPPORTED, "the residual operator is the cell-centred one");
   hipStream_t s = current_stream();
   int np = 0;
   FL_TRY(launch_absmax(f, h->nx, h->ny, h->nz, h->part, &np, s));

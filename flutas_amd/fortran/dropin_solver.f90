@@ -8,7 +8,7 @@
 !   fftini      src/fft.f90:24-157         the x and y r2r plans and normfft
 !   fft         src/fft.f90:181-193        (find_fft's table, src/fft.f90:233-291)
 !               on their own, for callers of the transforms; executed on
-!               the GPU.  Face-centred NN/DD (REDFT00/RODFT00) stop.
+!               the GPU, every kind of the table included.
 !   fftend      src/fft.f90:159-179        destroys either kind of arrplan.
 !   solver_gpu  src/solver_gpu.f90:31-47   solves in place on the interior
 !   solver_cpu  src/solver_cpu.f90:20-34   of p; the transforms, eigenvalues
@@ -101,8 +101,7 @@ module mod_fft
   type(c_ptr), allocatable, save :: issued(:)
 contains
   ! Transform kinds and normalisation of one direction (find_fft,
-  ! src/fft.f90:233-291).  The face-centred NN and DD kinds (REDFT00,
-  ! RODFT00) are not built and stop the program.
+  ! src/fft.f90:233-291).
   subroutine find_fft(bc, c_or_f, kind_fwd, kind_bwd, norm)
     character(len=1), intent(in ), dimension(0:1) :: bc
     character(len=1), intent(in )                 :: c_or_f
@@ -124,7 +123,8 @@ contains
       case ('PP'); kind_fwd = R2HC;    kind_bwd = HC2R;    norm = [1.0_rp, 0.0_rp]
       case ('ND'); kind_fwd = REDFT10; kind_bwd = REDFT01; norm = [2.0_rp, 0.0_rp]
       case ('DN'); kind_fwd = RODFT01; kind_bwd = RODFT10; norm = [2.0_rp, 0.0_rp]
-      case ('NN', 'DD'); error stop 'flutas_amd fftini: face-centred NN/DD (REDFT00/RODFT00) is not built'
+      case ('NN'); kind_fwd = REDFT00; kind_bwd = REDFT00; norm = [2.0_rp, -1.0_rp]
+      case ('DD'); kind_fwd = RODFT00; kind_bwd = RODFT00; norm = [2.0_rp, 1.0_rp]
       case default; error stop 'flutas_amd fftini: unknown boundary-condition pair'
       end select
     else
@@ -166,7 +166,8 @@ contains
   !
   ! fftini (src/fft.f90:24-157): the forward and backward r2r plans of x
   ! (arrplan(:,1), over n_x arrays) and y (arrplan(:,2), over n_y arrays) and
-  ! normfft = 1 / prod(norm(1) * (n + norm(2))).
+  ! normfft = 1 / prod(norm(1) * (n + norm(2) - i)).  A face-centred DD
+  ! direction transforms n - 1 points (i = 1): its last face is the boundary.
   subroutine fftini(n_x,n_y,bcxy,c_or_f,arrplan,normfft)
     integer         , intent(in ), dimension(3)     :: n_x,n_y
     character(len=1), intent(in ), dimension(0:1,2) :: bcxy
@@ -176,17 +177,22 @@ contains
     integer(c_int) :: kf, kb
     real(rp) :: norm(2)
     integer(c_size_t) :: ex, ey
+    integer :: ix, iy
     ex = int(n_x(1), c_size_t)*n_x(2)*n_x(3)
     ey = int(n_y(1), c_size_t)*n_y(2)*n_y(3)
     normfft = 1.0_rp
+    ix = 0
+    if (bcxy(0,1)//bcxy(1,1) == 'DD' .and. c_or_f(1) == 'f') ix = 1
     call find_fft(bcxy(:,1), c_or_f(1), kf, kb, norm)
-    arrplan(1,1) = new_plan(kf, n_x(1), 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
-    arrplan(2,1) = new_plan(kb, n_x(1), 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
-    normfft = normfft*norm(1)*(n_x(1) + norm(2))
+    arrplan(1,1) = new_plan(kf, n_x(1) - ix, 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
+    arrplan(2,1) = new_plan(kb, n_x(1) - ix, 1, n_x(2), n_x(1), n_x(3), n_x(1)*n_x(2), ex)
+    normfft = normfft*norm(1)*(n_x(1) + norm(2) - ix)
+    iy = 0
+    if (bcxy(0,2)//bcxy(1,2) == 'DD' .and. c_or_f(2) == 'f') iy = 1
     call find_fft(bcxy(:,2), c_or_f(2), kf, kb, norm)
-    arrplan(1,2) = new_plan(kf, n_y(2), n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
-    arrplan(2,2) = new_plan(kb, n_y(2), n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
-    normfft = normfft*norm(1)*(n_y(2) + norm(2))
+    arrplan(1,2) = new_plan(kf, n_y(2) - iy, n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
+    arrplan(2,2) = new_plan(kb, n_y(2) - iy, n_y(1), n_y(1), 1, n_y(3), n_y(1)*n_y(2), ey)
+    normfft = normfft*norm(1)*(n_y(2) + norm(2) - iy)
     normfft = normfft**(-1)
   end subroutine fftini
   !

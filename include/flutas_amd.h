@@ -65,9 +65,17 @@ int flutas_pointer_is_device(const void *ptr, int *is_device);
  * ng: global grid (one GPU = one rank); dli = 1/(dx,dy,dz);
  * dzci, dzfi: host arrays (1-nh_d : ng[2]+nh_d).  precision: FLUTAS_FP64 or
  * FLUTAS_FP32_REFINE; tol: relative residual target of the refined mode
- * (ignored for FP64).  Unsupported here (EUNSUPPORTED): c_or_f 'f'; odd ng
+ * (ignored for FP64).  c_or_f 'c' (cell-centred) or 'f' (face-centred) per
+ * direction, with find_fft's kinds (src/fft.f90:233-291), tridmatrix's and
+ * bc_rhs's 'f' branches (src/initsolver.f90:210-236, :277-287) and, for an
+ * 'f' z direction with a D top face, nz - 1 planes in the z system
+ * (src/solver_cpu.f90:71-77).  Unsupported here (EUNSUPPORTED): 'f' NN and
+ * DD in x or y (the REDFT00/RODFT00 solves, whose eigenvalues in
+ * src/initsolver.f90:164-178 belong to other transforms) and 'f' with
+ * FLUTAS_FP32_REFINE (its residual is the cell-centred operator); odd ng
  * (EINVAL, src/sanity.f90:155-158).  ND/DN pairs in x or y use REDFT11 /
- * RODFT11 on the five-pass line pipeline. */
+ * RODFT11 ('c') or RODFT01 forward ('f' DN) on the five-pass line
+ * pipeline. */
 int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
                          const char c_or_f[3], const double dli[3], int nh_d,
                          const double *dzci, const double *dzfi, int precision,
@@ -130,8 +138,12 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 /* fft (src/fft.f90:181-193): executes one r2r kind in place on `howmany`
  * device lines of length n (element stride `stride`, line distance
  * `dist`), FP64, unnormalised FFTW definitions.  kind uses FFTW's
- * numbering (src/fftw.f90:38-61); built: R2HC(0), HC2R(1), REDFT01(4),
- * REDFT10(5), REDFT11(6), RODFT01(8), RODFT10(9), RODFT11(10); n even. */
+ * numbering (src/fftw.f90:38-61); built: R2HC(0), HC2R(1), REDFT00(3),
+ * REDFT01(4), REDFT10(5), REDFT11(6), RODFT00(7), RODFT01(8), RODFT10(9),
+ * RODFT11(10).  n even, except for REDFT00 (n >= 2) and RODFT00 (any n),
+ * which run as the real DFT of their symmetric extension (2(n-1) or
+ * 2(n+1) points); a prime factor of n/2 (or of the extension's half
+ * length) above 1021 returns EUNSUPPORTED. */
 int flutas_fft_lines(int kind, int n, int howmany, long long stride,
                      long long dist, double *data);
 /* The same transform over a two-level batch, in the geometry of the guru

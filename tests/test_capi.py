@@ -109,9 +109,19 @@ def test_invalid_arguments_rejected_without_device():
     with pytest.raises(api.FlutasError) as e:
         api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PNPPNN")  # unpaired P
     assert e.value.code == api.FLUTAS_EINVAL
+    # face-centred NN / DD in x or y (REDFT00/RODFT00 solves) are not built,
+    # and the refined mode is cell-centred only
+    for cbc, cf in (("NNPPNN", "fcc"), ("PPDDNN", "cfc")):
+        with pytest.raises(api.FlutasError) as e:
+            api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, cbc, c_or_f=cf)
+        assert e.value.code == api.FLUTAS_EUNSUPPORTED
     with pytest.raises(api.FlutasError) as e:
-        api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", c_or_f="fff")
+        api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", c_or_f="ccf",
+                        precision=api.FP32_REFINE, tol=1e-10)
     assert e.value.code == api.FLUTAS_EUNSUPPORTED
+    with pytest.raises(api.FlutasError) as e:
+        api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", c_or_f="ccx")
+    assert e.value.code == api.FLUTAS_EINVAL
     with pytest.raises(api.FlutasError) as e:
         api.PoissonPlan((8, 8, 8), (1, 1, 1), dz, dz, "PPPPNN", precision=16)
     assert e.value.code == api.FLUTAS_EINVAL
@@ -203,7 +213,7 @@ FFT_DEMO = os.path.join(ROOT, "flutas_amd", "fortran", "_build", "fft_demo")
 
 
 @pytest.mark.parametrize("bcs", ["PPNNcc", "NNDDcc", "NDDNcc", "DDPPcc", "PPNDff", "DNPPfc",
-                                 "NDNNfc"])
+                                 "NDNNfc", "NNPPfc", "PPDDcf", "DDNNff"])
 def test_fortran_fftini_normfft(oracle, bcs):
     """The drop-in mod_fft's fftini (src/fft.f90:24-157) gives the oracle's
     fftini normalisation for nx = 12, ny = 10, face-centred pairs included;
@@ -221,13 +231,3 @@ def test_fortran_fftini_normfft(oracle, bcs):
     assert got == s.normfft, (got, s.normfft)
 
 
-@pytest.mark.parametrize("bcs", ["NNPPfc", "PPDDcf"])
-def test_fortran_fftini_rejects_face_nn_dd(bcs):
-    """Face-centred NN and DD (REDFT00/RODFT00) are not built: fftini stops
-    with a message instead of returning a plan."""
-    import subprocess
-    if not os.path.exists(FFT_DEMO):
-        pytest.skip("Fortran modules not built (make -C flutas_amd/fortran)")
-    r = subprocess.run([FFT_DEMO, "-n", bcs], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0
-    assert "REDFT00/RODFT00" in r.stdout + r.stderr

@@ -410,3 +410,121 @@ def test_correc_linear_pressure(oracle, alpha, const_coeff):
     scale = rho0 if const_coeff else 2.0
     for new, old, g in ((u, u0, ca), (v, v0, cb), (w, w0, cc)):
         np.testing.assert_allclose(interior(old) - interior(new), dt * g / scale, rtol=1e-10)
+
+
+# ---- face-centred ('f') pieces ---------------------------------------------
+
+def test_tridmatrix_face_uniform_closed_form(oracle):
+    """src/initsolver.f90:210-236, 'f': a = c = 1/dz^2 on a uniform grid; an
+    N face adds a(1) or c(n) to the diagonal, a D face (a boundary value,
+    no unknown) leaves it at -2/dz^2."""
+    nz, lz = 8, 1.0
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, 0.0, lz)
+    h2 = (nz / lz) ** 2
+    for bc, (b0, b1) in (("NN", (-h2, -h2)), ("DD", (-2 * h2, -2 * h2)),
+                         ("ND", (-h2, -2 * h2)), ("DN", (-2 * h2, -h2)),
+                         ("PP", (-2 * h2, -2 * h2))):
+        a, b, c = oracle.tridmatrix(bc, nz, 1, dzci, dzfi, "f")
+        np.testing.assert_allclose(a, h2, rtol=1e-14)
+        np.testing.assert_allclose(c, h2, rtol=1e-14)
+        np.testing.assert_allclose(b[1:-1], -2 * h2, rtol=1e-14)
+        np.testing.assert_allclose([b[0], b[-1]], [b0, b1], rtol=1e-14)
+
+
+@pytest.mark.parametrize("alpha", [1.0, 2.0])
+def test_tridmatrix_face_stretched_invariants(oracle, alpha):
+    """Face unknowns at zf(k): the row of face k is the difference of the
+    fluxes through the cells k and k+1 over the centre distance dzc(k), so
+    dzc(k) c(k) = dzc(k+1) a(k+1) = 1/dzf(k+1), and the NN rows annihilate
+    constants."""
+    nz = 20
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, alpha, 1.0)
+    a, b, c = oracle.tridmatrix("NN", nz, 1, dzci, dzfi, "f")
+    wc = dzc[1:nz + 1]
+    np.testing.assert_allclose(wc[:-1] * c[:-1], 1.0 / dzf[2:nz + 1], rtol=1e-13)
+    np.testing.assert_allclose(wc[1:] * a[1:], 1.0 / dzf[2:nz + 1], rtol=1e-13)
+    m = tri_dense(a, b, c, False)
+    assert np.max(np.abs(m @ np.ones(nz))) < 1e-10 * np.max(np.abs(b))
+
+
+def _face_dn_1d(n, dli):
+    """Face-centred DN second difference: unknowns on nodes 1..n, the D
+    value 0 on node 0, a mirror p(n+1) = p(n-1) about the N node n."""
+    m = np.zeros((n, n))
+    for i in range(n):
+        m[i, i] = -2.0
+        if i > 0:
+            m[i, i - 1] = 1.0
+        if i < n - 1:
+            m[i, i + 1] = 1.0
+    m[n - 1, n - 2] = 2.0
+    return m * dli * dli
+
+
+@pytest.mark.parametrize("cbc,c_or_f,alpha", [
+    ("PPPPNN", "ccf", 1.3),
+    ("PPPPDD", "ccf", 1.0),
+    ("NNPPND", "ccf", 0.0),
+    ("DDNNDN", "ccf", 1.5),
+    ("DNPPNN", "fcc", 0.0),
+    ("PPDNDD", "cff", 0.0),
+])
+def test_face_solve_matches_dense_solve(oracle, cbc, c_or_f, alpha):
+    """The whole 'f' solve against a dense solve: z rows from the oracle's
+    'f' tridmatrix (pinned above), a D top face taking nz - 1 rows with the
+    last plane passed through (src/solver_cpu.f90:71-77), and the face DN
+    pair of x or y (RODFT01 forward, RODFT10 backward, src/fft.f90:283-286)
+    inverting the node-based DN second difference."""
+    ng = (6, 8, 10)
+    nx, ny, nz = ng
+    dzc, dzf, dzci, dzfi, _, _ = z_metrics(oracle, nz, alpha, 0.9)
+    dli = (nx / 1.0, ny / 1.3, nz / 0.9)
+    s = oracle.Solver(ng, dli, dzci, dzfi, cbc, c_or_f=c_or_f)
+    q = 1 if (c_or_f[2] == "f" and cbc[5] == "D") else 0
+    m = nz - q
+
+    def op1(n, bc, cf, d):
+        if cf == "f" and bc == "DN":
+            return _face_dn_1d(n, d)
+        assert cf == "c" or bc == "PP"
+        return second_diff_1d(n, bc, d)
+
+    lx = op1(nx, cbc[0:2], c_or_f[0], s.dli[0])
+    ly = op1(ny, cbc[2:4], c_or_f[1], s.dli[1])
+    lz = tri_dense(s.a[:m], s.b[:m], s.c[:m], cbc[4:6] == "PP")
+    ix, iy, iz = np.eye(nx), np.eye(ny), np.eye(m)
+    L = (np.kron(iz, np.kron(iy, lx)) + np.kron(iz, np.kron(ly, ix)) +
+         np.kron(lz, np.kron(iy, ix)))
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(ng)
+    if all(ch in "PN" for ch in cbc) and q == 0:
+        # singular: make f compatible with the left null vector of L
+        # (constant in x and y, that of the z rows in z)
+        _, _, vt = np.linalg.svd(lz.T)
+        wz = vt[-1]
+        f -= np.sum(f * wz[None, None, :]) / (nx * ny * np.sum(wz))
+    p = halo_field(ng, 1)
+    p[1:-1, 1:-1, 1:-1] = f
+    s.solve(p, pin_gauge=True)
+    got = interior(p)
+    fv = f[:, :, :m].reshape(-1, order="F")
+    res = L @ got[:, :, :m].reshape(-1, order="F") - fv
+    assert np.max(np.abs(res)) < 1e-11 * np.max(np.abs(fv))
+    if q:
+        # the boundary plane goes through the transform pair unchanged
+        np.testing.assert_allclose(got[:, :, nz - 1], f[:, :, nz - 1], rtol=0, atol=1e-13)
+
+
+def test_face_nd_pair_is_the_references(oracle):
+    """The face-centred ND pair as the reference selects it: REDFT10 forward,
+    REDFT01 backward, norm (2, 0) (src/fft.f90:279-282), with the ND
+    eigenvalues of src/initsolver.f90:180-183.  These are the DCT-II/III of
+    the cell-centred NN pair, so the solve inverts no second difference
+    (the pairing's own behaviour; its components are pinned by the r2r
+    golden vectors and the ND spectrum test)."""
+    kf, kb, norm = oracle.find_fft("ND", "f")
+    assert (kf, kb, norm) == (oracle.KIND["REDFT10"], oracle.KIND["REDFT01"], (2.0, 0.0))
+    np.testing.assert_array_equal(oracle.eigenvalues(12, "ND", "f"),
+                                  oracle.eigenvalues(12, "ND", "c"))
+    kf, kb, norm = oracle.find_fft("DN", "f")
+    assert (kf, kb, norm) == (oracle.KIND["RODFT01"], oracle.KIND["RODFT10"], (2.0, 0.0))

@@ -244,6 +244,59 @@ __global__ void __launch_bounds__(kRedThreads)
   }
 }
 
+// Final reduction of the nb block partials of k_chkdiv (one workgroup, a
+// fixed order, so repeated calls on the same field give the same bits):
+// out[0] = divtot, out[1] = divmax (NaN propagating).
+__global__ void __launch_bounds__(kRedThreads)
+    k_chkdiv_final(const double *__restrict__ part, int nb, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double ssum[kRedThreads], smax[kRedThreads];
+  double s = 0.0, m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+    s += part[2 * b];
+    const double o = part[2 * b + 1];
+    m = (o > m || o != o) ? o : m;
+  }
+  ssum[threadIdx.x] = s;
+  smax[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + off];
+      const double o = smax[threadIdx.x + off];
+      if (o > smax[threadIdx.x] || o != o) smax[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = ssum[0];
+    out[1] = smax[0];
+  }
+}
+
+// Partial count of a chkdiv launch: one per 256 points, at most 2048 (the
+// kernel strides over the grid, so any grid fits).
+constexpr int kChkdivBlocks = 2048;
+static int chkdiv_blocks(long long total) {
+  return (int)std::min<long long>(kChkdivBlocks, (total + kRedThreads - 1) / kRedThreads);
+}
+
+// Enqueues chkdiv on stream s with the partials and the two results in
+// `work` (2 * chkdiv_blocks + 2 doubles, results last) and copies the
+// results to the host values; the caller synchronises.
+static int chkdiv_enqueue(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
+                          const double *dzfi, const double *u, const double *v,
+                          const double *w, double *work, double *res_host, hipStream_t s) {
+  const int nb = chkdiv_blocks((long long)nx * ny * nz);
+  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(kRedThreads), 0, s, nx, ny, nz, nh_d, nh_u, dxi,
+                     dyi, dzfi, u, v, w, work);
+  FL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(kRedThreads), 0, s, work, nb, work + 2 * nb);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(res_host, work + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  return FLUTAS_OK;
+}
+
 // ---- boundp / set_bc, nh_p = 1 (src/bound.f90:146-646) -------------------
 // One face: dir 0/1/2, side 0/1; type 'P' periodic copy, 'D' centred
 // Dirichlet p0 = 2v - p1, 'N' p0 = p1 -+ dr v.  Loops over the full
@@ -793,26 +846,17 @@ int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
   (void)dzi;
   FL_TRY(check_dims(nx, ny, nz));
   if (!dzfi || !u || !v || !w || !divtot || !divmax) return fail(FLUTAS_EINVAL, "null pointer");
-  const long long total = (long long)nx * ny * nz;
-  const int nb = (int)std::min<long long>(2048, (total + kRedThreads - 1) / kRedThreads);
-  double *part = nullptr;
-  FL_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * nb, current_stream()));
-  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(kRedThreads), 0, current_stream(),
-                     nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, part);
-  FL_HIP(hipGetLastError());
-  std::vector<double> h(2 * nb);
-  FL_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb,
-                        hipMemcpyDeviceToHost, current_stream()));
-  FL_HIP(hipFreeAsync(part, current_stream()));
-  FL_HIP(hipStreamSynchronize(current_stream()));
-  double s = 0.0, m = 0.0;
-  for (int b = 0; b < nb; ++b) {
-    s += h[2 * b];
-    const double o = h[2 * b + 1];
-    if (o > m || o != o) m = o;
-  }
-  *divtot = s;
-  *divmax = m;
+  hipStream_t s = current_stream();
+  const int nb = chkdiv_blocks((long long)nx * ny * nz);
+  double *work = nullptr;
+  FL_HIP(hipMallocAsync((void **)&work, sizeof(double) * (2 * nb + 2), s));
+  double res[2] = {0.0, 0.0};
+  const int rc = chkdiv_enqueue(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, work, res, s);
+  FL_HIP(hipFreeAsync(work, s));
+  FL_HIP(hipStreamSynchronize(s));
+  if (rc) return rc;
+  *divtot = res[0];
+  *divmax = res[1];
   return FLUTAS_OK;
 }
 

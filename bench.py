@@ -172,8 +172,10 @@ def step_bench(api, case, dzc, dzf, dzci, dzfi, f_host, steps, warmup):
 def batch_bench(api, wl, reps=5):
     """The uneven batch (64 problems, nx, ny in {64..256}, nz in {64, 128,
     256}, three BC patterns, half stretched): aggregate grid points per
-    second of the batched graph (before and after autotuning) and of the
-    same solves run one after another, wall clock per round."""
+    second of the persistent batched launch, plain FP64 and FP32 + FP64
+    refinement to 1e-10 (before and after autotuning), against the same 64
+    problems solved one after another with single plain FP64 plans; wall
+    clock per round."""
     cases = wl.batch_problems()
     problems, fs, ps, singles = [], [], [], []
     npts = 0
@@ -186,8 +188,6 @@ def batch_bench(api, wl, reps=5):
         fs.append(f)
         ps.append(p)
         npts += case.npoints
-    batch = api.BatchPlan(problems)
-    batch.set_fields(fs, ps)
 
     def rate(fn):
         fn()
@@ -199,13 +199,32 @@ def batch_bench(api, wl, reps=5):
         dt = (time.perf_counter() - t0) / reps
         return dt * 1e3, npts / dt
 
-    ms_b, pps_b = rate(batch.solve)
-    t0 = time.perf_counter()
-    batch.tune(reps=2)
-    tune_s = time.perf_counter() - t0
-    ms_t, pps_t = rate(batch.solve)
-    th, gh = batch.tuning()
-    batch.destroy()
+    out = {"problems": len(cases), "grid_points": npts}
+    for key, prec in (("fp64", api.FP64), ("refined", api.FP32_REFINE)):
+        batch = api.BatchPlan(problems, precision=prec, tol=1e-10)
+        batch.set_fields(fs, ps)
+        ms_b, pps_b = rate(batch.solve)
+        t0 = time.perf_counter()
+        batch.tune(reps=2)
+        tune_s = time.perf_counter() - t0
+        ms_t, pps_t = rate(batch.solve)
+        th, gh = batch.tuning()
+        wg, items, per_cu, lds = batch.launch_info()
+        shape, kxp = batch.launch_shape()
+        shape = ("launch per pass and line length", "persistent launch per pass")[shape]
+        res = {"untuned_ms": ms_b, "untuned_points_per_s": pps_b, "launch_shape": shape,
+               "x_item_planes": kxp,
+               "tuned_ms": ms_t, "tuned_points_per_s": pps_t, "tune_seconds": tune_s,
+               "tile_variants": {str(v): th.count(v) for v in sorted(set(th))},
+               "group_variants": {str(v): gh.count(v) for v in sorted(set(gh))},
+               "workgroups": wg, "queue_items": items, "workgroups_per_cu": per_cu,
+               "lds_bytes": lds}
+        if prec == api.FP32_REFINE:
+            iters, rel = batch.stats()
+            res["rounds"] = {str(v): iters.count(v) for v in sorted(set(iters))}
+            res["worst_relres"] = max(rel)
+        batch.destroy()
+        out[key] = res
     for (ng, dli, dzci, dzfi, cbc) in problems:
         singles.append(api.PoissonPlan(ng, dli, dzci, dzfi, cbc))
 
@@ -216,12 +235,10 @@ def batch_bench(api, wl, reps=5):
     ms_s, pps_s = rate(sequential)
     for plan in singles:
         plan.destroy()
-    return {"problems": len(cases), "grid_points": npts,
-            "batched_ms": ms_b, "batched_points_per_s": pps_b,
-            "tuned_ms": ms_t, "tuned_points_per_s": pps_t, "tune_seconds": tune_s,
-            "tuned_half_tile": sum(th), "tuned_half_group": sum(gh),
-            "sequential_ms": ms_s, "sequential_points_per_s": pps_s,
-            "speedup_vs_sequential": ms_s / ms_t}
+    out.update({"sequential_fp64_ms": ms_s, "sequential_fp64_points_per_s": pps_s,
+                "fp64_speedup_vs_sequential": ms_s / out["fp64"]["tuned_ms"],
+                "refined_speedup_vs_sequential": ms_s / out["refined"]["tuned_ms"]})
+    return out
 
 
 def main():
@@ -352,9 +369,10 @@ def main():
         log("bench: uneven batch of 64 problems ...")
         result["batch"] = batch_bench(api, wl)
         bb = result["batch"]
-        log(f"bench: batch {bb['grid_points'] / 1e6:.1f} M pts: graph {bb['batched_ms']:.2f} ms, "
-            f"tuned {bb['tuned_ms']:.2f} ms, sequential {bb['sequential_ms']:.2f} ms "
-            f"({bb['speedup_vs_sequential']:.2f}x)")
+        log(f"bench: batch {bb['grid_points'] / 1e6:.1f} M pts: FP64 {bb['fp64']['tuned_ms']:.2f} ms "
+            f"(untuned {bb['fp64']['untuned_ms']:.2f}), refined {bb['refined']['tuned_ms']:.2f} ms "
+            f"(untuned {bb['refined']['untuned_ms']:.2f}), sequential FP64 "
+            f"{bb['sequential_fp64_ms']:.2f} ms")
     if args.calibrate_copy:
         del f, p
         a = api.DeviceArray((COPY_N + 2,) * 3)

@@ -247,12 +247,14 @@ class PoissonPlan:
 
 
 class BatchPlan:
-    """Many independent FP64 problems solved together (flutas_batch_*):
-    one launch per pass over all problems, captured in a hipGraph.
+    """Many independent problems solved together (flutas_batch_*): one
+    persistent launch over a work queue of every pass of every problem,
+    captured in a hipGraph.
 
-    problems: sequence of (ng, dli, dzci, dzfi, cbc) with nh_d = 1."""
+    problems: sequence of (ng, dli, dzci, dzfi, cbc) with nh_d = 1.
+    precision: FP64, or FP32_REFINE with the relative residual target tol."""
 
-    def __init__(self, problems, nh_d=1):
+    def __init__(self, problems, nh_d=1, precision=FP64, tol=1e-10):
         self.problems = [(tuple(int(v) for v in ng), tuple(float(v) for v in dli),
                           np.ascontiguousarray(dzci, np.float64),
                           np.ascontiguousarray(dzfi, np.float64), _bc6(cbc))
@@ -265,9 +267,11 @@ class BatchPlan:
         dzci = (_D * n)(*[pr[2].ctypes.data_as(_D) for pr in self.problems])
         dzfi = (_D * n)(*[pr[3].ctypes.data_as(_D) for pr in self.problems])
         h = _VP()
-        _check(lib.flutas_batch_create(C.c_int(n), ng, C.c_char_p(cbc), dli, C.c_int(nh_d),
-                                       dzci, dzfi, C.byref(h)))
+        _check(lib.flutas_batch_create_ex(C.c_int(n), ng, C.c_char_p(cbc), dli, C.c_int(nh_d),
+                                          dzci, dzfi, C.c_int(precision), C.c_double(tol),
+                                          C.byref(h)))
         self._h = h.value
+        self.precision = precision
         self._fields = None
 
     def set_fields(self, fs, ps):
@@ -287,14 +291,38 @@ class BatchPlan:
     def solve(self):
         _check(lib.flutas_batch_solve(_VP(self._h)))
 
+    def stats(self):
+        """Per problem: (correction rounds, relative residual) of the last solve."""
+        n = len(self.problems)
+        it, rr = (C.c_int * n)(), (C.c_double * n)()
+        _check(lib.flutas_batch_stats(_VP(self._h), it, rr))
+        return list(it), list(rr)
+
     def tune(self, reps=3):
         _check(lib.flutas_batch_tune(_VP(self._h), C.c_int(reps)))
 
     def tuning(self):
+        """Per problem: (x-tile variant, plane-group variant), 0 full, 1 half,
+        2 quarter."""
         n = len(self.problems)
         th, gh = (C.c_int * n)(), (C.c_int * n)()
         _check(lib.flutas_batch_tuning(_VP(self._h), th, gh))
         return list(th), list(gh)
+
+    def launch_shape(self):
+        """(shape, planes per x-tile item); shape 0: one launch per pass and
+        line length, 1: one persistent launch per pass."""
+        v, k = C.c_int(), C.c_int()
+        _check(lib.flutas_batch_launch_shape(_VP(self._h), C.byref(v), C.byref(k)))
+        return v.value, k.value
+
+    def launch_info(self):
+        """(workgroups, queue items, workgroups per CU, dynamic LDS bytes) of
+        the persistent launch."""
+        wg, it, pc, lds = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        _check(lib.flutas_batch_launch_info(_VP(self._h), C.byref(wg), C.byref(it),
+                                            C.byref(pc), C.byref(lds)))
+        return wg.value, it.value, pc.value, lds.value
 
     def destroy(self):
         if getattr(self, "_h", None):

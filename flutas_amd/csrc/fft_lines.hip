@@ -522,7 +522,8 @@ int launch_lines(const LineTables &tab, int kind, const Tin *in, Tout *out,
   const size_t shmem = per_line * lpb;
   LinePlan<T> pl;
   if constexpr (sizeof(T) == 8) pl = tab.dplan(); else pl = tab.fplan();
-  if (m != g.n)
+  // (REDFT00 of two points has m == n: dispatch on the kind)
+  if (kind == K_REDFT00 || kind == K_RODFT00)
     hipLaunchKernelGGL((k_lines_ext<T, Tin, Tout>), grid, dim3(256), shmem, s, in,
                        out, g, pl, kind, scale, lpb);
   else

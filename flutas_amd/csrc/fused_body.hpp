@@ -64,6 +64,12 @@ struct Tw {
   static constexpr int total = ntw + nth + ntq;
   const T *tw, *th, *tq;
 };
+// The staged tables of a previous stage_twiddles call into s.
+template <typename T, int NC>
+__device__ __forceinline__ Tw<T, NC> twiddle_view(T *s) {
+  using W = Tw<T, NC>;
+  return Tw<T, NC>{s, s + W::ntw, s + W::ntw + W::nth};
+}
 template <typename T, int NC, int NT>
 __device__ __forceinline__ Tw<T, NC> stage_twiddles(T *s, const T *__restrict__ tw,
                                                     const T *__restrict__ twh,
@@ -154,7 +160,8 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
                                            const T *__restrict__ twh,
                                            const T *__restrict__ twq, double *absmax_slot,
                                            T *sm, T *stw, int j0, int k,
-                                           int *nf_flag = nullptr, int nf_seq = 0) {
+                                           int *nf_flag = nullptr, int nf_seq = 0,
+                                           bool stage_tw = true) {
   constexpr int n = 2 * NC;  // nx
   constexpr int LS = line_stride<T, n>();
   constexpr int LD = (TJ * n + NT - 1) / NT;
@@ -181,7 +188,8 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
     }
     sm[l * LS + pos] = x;
   }
-  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  const Tw<T, NC> W =
+      stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw);
   if (absmax_slot) {
     // max |in| over the tile (the clamped loads only repeat tile entries),
     // NaN propagating
@@ -218,7 +226,7 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
                                            T scale, const T *__restrict__ tw,
                                            const T *__restrict__ twh,
                                            const T *__restrict__ twq, T *sm, T *stw, int j0,
-                                           int k) {
+                                           int k, bool stage_tw = true) {
   constexpr int n = 2 * NC;
   constexpr int LS = line_stride<T, n>();
   constexpr int LD = (TJ * n + NT - 1) / NT;
@@ -248,7 +256,8 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
     const int t = threadIdx.x + q * NT;
     if (t < TJ * n) sm[(t % TJ) * LS + t / TJ] = r[q];
   }
-  const Tw<T, NC> W = stage_twiddles<T, NC, NT>(stw, tw, twh, twq);
+  const Tw<T, NC> W =
+      stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw);
   __syncthreads();
   tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
   tile::cfft<T, +1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);

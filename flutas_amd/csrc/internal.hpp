@@ -166,43 +166,76 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int periodic, hipStream_t s);
 
-// ---- batched fused pipeline (many independent FP64 problems) ------------
-// One launch per stage and line length covers every problem of that length;
-// a workgroup reads its (problem, tile) from the item table and runs the
-// body compiled for that problem's line lengths.
+// ---- batched pipeline: one persistent launch over a work queue ---------
+// Stages of the batched solve (see batch.hip): the four passes of a solve,
+// then, in the refined mode, the residual, the four passes of the
+// correction solve and the residual check.
+enum BatchStage {
+  BS_X_FWD = 0, BS_YZ_FWD = 1, BS_ZY_BWD = 2, BS_X_BWD = 3,
+  BS_RESID = 4, BS_X_FWD_R = 5, BS_YZ_FWD_R = 6, BS_ZY_BWD_R = 7,
+  BS_X_BWD_ACC = 8, BS_CHECK = 9
+};
+constexpr int kBatchStages = 10;
 struct BatchProblem {
   int nx, ny, nz;
-  int ncx, ncy;                    // nx / 2, ny / 2 (compiled plan selector)
   int kx_f, kx_b, ky_f, ky_b;      // r2r kinds of the four passes
   int per, pin;                    // periodic z; singular column pinned
-  int tj_half, gp_half;            // tuned variants: half tile / half group
+  int tj_v, gp_v;                  // tuned variants (0 full, 1 half, 2 quarter)
+  int kxp;                         // planes of one x-tile item (tuned)
+  int active;                      // refined: still above its tolerance
+  int rpi;                         // residual items: rows per item
+  int cnt[kBatchStages];           // items of each stage (completion target)
+  char cbc[8];                     // boundary conditions (residual operator)
   double scale;                    // normfft
+  double dx2, dy2;                 // dli^2 (residual operator)
   long long hx, hxy;               // strides of the halo-1 fields
   const double *f;                 // right-hand side (interior origin)
   double *p;                       // solution (interior origin)
-  double *W, *ck;                  // work field, z checkpoints
+  void *W;                         // work field (FP64, or FP32 refined)
+  float *R;                        // residual (refined, dense nx*ny*nz)
+  double *ck;                      // z checkpoints
   const double *lamx, *lamy, *a, *b, *c;
   const double *twx, *twhx, *twqx, *twy, *twhy, *twqy;
+  const float *ftwx, *ftwhx, *ftwqx, *ftwy, *ftwhy, *ftwqy;
 };
-struct BatchItem {
-  int prob, u, v;  // x passes: (tile, plane); slab passes: (slab, 0)
+struct BatchWork {
+  int stage, prob, u, v;  // x stages: (tile, plane); slabs: (slab, 0);
+                          // residual: (row group, plane chunk)
 };
-enum { BATCH_X_FWD = 0, BATCH_YZ_FWD = 1, BATCH_ZY_BWD = 2, BATCH_X_BWD = 3 };
+// Arguments of one pass launch: the pass's items and the per-problem
+// maxima ([3 prob + m] bits of max |f|, |r| after the solve, |r| after the
+// correction).
+struct BatchPassArgs {
+  const BatchProblem *P;
+  const BatchWork *items;
+  int nitems;
+  unsigned long long *mx;
+};
 // line lengths the batched kernels are compiled for
 bool batch_supported(int n);
-// lines per x tile and planes per slab group of a length (FP64), full or
-// halved variant
-int batch_tile_lines(int nx, int half);
-int batch_group_planes(int ny, int half);
-// dynamic LDS bytes of a stage for line length n (the largest variant)
-size_t batch_lds_bytes(int stage, int n);
-// raises the dynamic-LDS limit of the batched kernels (call before
-// launching or capturing them)
-int batch_set_lds(const size_t lds_bytes[4]);
-// one launch of a stage over nitems items whose line length (nx for the x
-// stages, ny for the slab stages) is n
-int launch_batch_stage(int stage, const BatchProblem *probs, const BatchItem *items,
-                       int nitems, size_t lds_bytes, int n, hipStream_t s);
+// lines per x tile and planes per slab group of a length and variant
+int batch_tile_lines(int nx, int elem_bytes, int v);
+int batch_group_planes(int ny, int elem_bytes, int v);
+// residual items: rows per item (nx <= 256), planes per item
+int batch_resid_rows(int nx);
+int batch_resid_planes();
+// dynamic LDS of one item body: stage_kind 0 x tile (length n = nx),
+// 1 slab (n = ny)
+size_t batch_item_lds(int stage_kind, int n, int elem_bytes, int v);
+// raises the dynamic-LDS limit of the pass kernel and returns the
+// workgroups per CU it reaches
+int batch_persistent_setup(int elem_bytes, size_t lds_bytes, int *blocks_per_cu);
+int launch_batch_pass(int elem_bytes, const BatchPassArgs &A, int grid, size_t lds_bytes,
+                      hipStream_t s);
+// per-length launch shape: one launch of stage `stage` over items whose
+// line length (nx for x stages, ny for slab stages) is n, one workgroup per
+// item; the residual stages take every problem's items in one launch
+int launch_batch_segment(int elem_bytes, const BatchPassArgs &A, int stage, int n,
+                         size_t lds_bytes, hipStream_t s);
+// raises the dynamic-LDS limit of that launch's kernel
+int batch_segment_setup(int elem_bytes, int stage, int n, size_t lds_bytes);
+// out[0..n) = maxima of the solve (bits in mx), mx cleared
+int launch_batch_publish(unsigned long long *mx, double *out, int n, hipStream_t s);
 
 // ---- refinement helpers ----------------------------------------------------
 struct OperatorDesc {

@@ -860,53 +860,73 @@ int flutas_initgrid_inivel(const char *inivel, int n, double gr, double lz,
 
 // ---- batched solves ---------------------------------------------------------
 // No counterpart in the reference (one solve per call there).  Each problem
-// keeps its own FP64 plan (coefficients, twiddles, work field); the batch
-// adds a device descriptor table, the item tables of the four passes, its
-// own checkpoint buffers (sized for the halved plane group) and a stream on
-// which the four launches are captured once into a hipGraph and replayed.
+// keeps its own plan (coefficients, twiddles, work field; FP64, or FP32 for
+// the refined mode); the batch adds a device descriptor table, its own
+// checkpoint buffers (sized for the quarter plane group), one item list per
+// pass (batch.hip) and the per-problem maxima.  Graph 0 is the whole solve
+// (FP64: four pass launches; refined: solve, residual, correction solve,
+// check and the publishing launch), graph 1 one more correction round of
+// the refined mode; each is captured once and replayed.
 struct flutas_batch {
   int nprob = 0;
+  int precision = FLUTAS_FP64;
+  double tol = 0.0;
+  int max_rounds = 8;
+  int elem = 8;  // bytes per value of the work fields
   std::vector<flutas_psolve *> plans;
   std::vector<BatchProblem> probs;
   std::vector<double *> ck;
   BatchProblem *d_probs = nullptr;
-  BatchItem *d_items[4] = {nullptr, nullptr, nullptr, nullptr};
-  int nitems[4] = {0, 0, 0, 0};
-  size_t lds[4] = {0, 0, 0, 0};
-  // Each pass is launched once per line length (a segment of its item
-  // table), with the dynamic LDS that length needs: a launch sized for the
-  // largest problem would hold every small problem to its occupancy.
+  BatchWork *d_items[kBatchStages] = {};
+  int nitems[kBatchStages] = {};
+  int grid[kBatchStages] = {};
+  // launch shape: 0 one launch per pass and line length (segments of the
+  // pass's item list, run side by side on the side streams), 1 one
+  // persistent launch per pass walking the whole list
+  int mode = 0;
   struct Segment {
-    int off, cnt;
+    int off, cnt, n;
     size_t lds;
-    int n;  // line length of every item of the segment
   };
-  std::vector<Segment> segs[4];
-  // Segments of one pass are independent: the captured graph runs them on
-  // side streams in parallel (fork and join through events), so the
-  // workgroups of short-line problems fill the CUs that a long-line
-  // segment leaves free.  Default stream priority.
-  static constexpr int kSide = 4, kEvents = 16;
+  std::vector<Segment> segs[kBatchStages];
+  static constexpr int kSide = 4, kEvents = 8;
   hipStream_t side[kSide] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t ev[4][kEvents] = {};
+  hipEvent_t ev[kBatchStages][kEvents] = {};
+  size_t lds = 0;
+  int ncu = 0, per_cu = 0, max_per_cu = 1;  // workgroups per CU (0: occupancy)
+  unsigned long long *d_mx = nullptr;       // maxima bits, 3 per problem
+  double *d_out = nullptr;                  // published maxima
+  double *h_out = nullptr;                  // pinned copy
   hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   bool fields_set = false;
+  std::vector<int> iters;
+  std::vector<double> relres;
 };
 
+static void batch_drop_graphs(flutas_batch *b) {
+  for (int g = 0; g < 2; ++g) {
+    if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
+    if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
+    b->exec[g] = nullptr;
+    b->graph[g] = nullptr;
+  }
+}
+
 static void batch_release(flutas_batch *b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  batch_drop_graphs(b);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   for (hipStream_t t : b->side)
     if (t) (void)hipStreamDestroy(t);
   for (auto &row : b->ev)
     for (hipEvent_t e : row)
       if (e) (void)hipEventDestroy(e);
-  for (BatchItem *d : b->d_items)
+  for (BatchWork *d : b->d_items)
     if (d) (void)hipFree(d);
-  if (b->d_probs) (void)hipFree(b->d_probs);
+  for (void *d : {(void *)b->d_probs, (void *)b->d_mx, (void *)b->d_out})
+    if (d) (void)hipFree(d);
+  if (b->h_out) (void)hipHostFree(b->h_out);
   for (double *c : b->ck)
     if (c) (void)hipFree(c);
   for (flutas_psolve *h : b->plans) flutas_psolve_destroy(h);
@@ -919,101 +939,229 @@ static int batch_upload(flutas_batch *b) {
   return FLUTAS_OK;
 }
 
-// Item tables: x passes take every (tile, plane) of every problem (tiles of
-// the problem's current tile size); slab passes take every slab, the
-// problems with the longest slabs first.  A new table drops the captured
-// graph, whose launch sizes it fixes.
-static int batch_items(flutas_batch *b) {
-  std::vector<BatchItem> items[4];
-  // problems by line length of the pass (nx for the x passes, ny for the
-  // slab passes), longest first; within a length, the slab passes take the
-  // problems with the most planes per slab first
-  std::vector<int> ox(b->nprob), oy(b->nprob);
-  for (int q = 0; q < b->nprob; ++q) ox[q] = oy[q] = q;
-  std::stable_sort(ox.begin(), ox.end(), [&](int x, int y) { return b->probs[x].nx > b->probs[y].nx; });
-  std::stable_sort(oy.begin(), oy.end(), [&](int x, int y) {
-    const BatchProblem &px = b->probs[x], &py = b->probs[y];
-    if (px.ny != py.ny) return px.ny > py.ny;
-    return (long long)px.nz * px.ny > (long long)py.nz * py.ny;
-  });
-  for (int st = 0; st < 4; ++st) b->segs[st].clear();
-  auto segment = [&](int st, int n, int off) {
-    auto &v = b->segs[st];
-    if (v.empty() || v.back().n != n) v.push_back({off, 0, batch_lds_bytes(st, n), n});
-    v.back().cnt = (int)items[st].size() - v.back().off;
-  };
-  for (int q : ox) {
-    const BatchProblem &p = b->probs[q];
-    const int tj = batch_tile_lines(p.nx, p.tj_half);
-    const int ntile = (p.ny + tj - 1) / tj;
-    const int off = (int)items[BATCH_X_FWD].size();
-    for (int k = 0; k < p.nz; ++k)
-      for (int t = 0; t < ntile; ++t) items[BATCH_X_FWD].push_back({q, t, k});
-    segment(BATCH_X_FWD, p.nx, off);
-  }
-  for (int q : oy) {
-    const int off = (int)items[BATCH_YZ_FWD].size();
-    for (int i = 0; i < b->probs[q].nx; ++i) items[BATCH_YZ_FWD].push_back({q, i, 0});
-    segment(BATCH_YZ_FWD, b->probs[q].ny, off);
-  }
-  items[BATCH_ZY_BWD] = items[BATCH_YZ_FWD];
-  items[BATCH_X_BWD] = items[BATCH_X_FWD];
-  b->segs[BATCH_ZY_BWD] = b->segs[BATCH_YZ_FWD];
-  b->segs[BATCH_X_BWD] = b->segs[BATCH_X_FWD];
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
-  for (int st = 0; st < 4; ++st) {
-    if (b->d_items[st]) (void)hipFree(b->d_items[st]);
-    b->d_items[st] = nullptr;
-    b->nitems[st] = (int)items[st].size();
-    FL_HIP(hipMalloc((void **)&b->d_items[st], items[st].size() * sizeof(BatchItem)));
-    FL_HIP(hipMemcpy(b->d_items[st], items[st].data(), items[st].size() * sizeof(BatchItem),
-                     hipMemcpyHostToDevice));
-  }
-  return FLUTAS_OK;
+// stages of graph g
+static void batch_stage_range(const flutas_batch *b, int g, int *s0, int *s1) {
+  if (b->precision == FLUTAS_FP64) { *s0 = BS_X_FWD; *s1 = BS_X_BWD; }
+  else if (g == 0) { *s0 = BS_X_FWD; *s1 = BS_CHECK; }
+  else { *s0 = BS_RESID; *s1 = BS_CHECK; }
 }
 
-static int batch_enqueue(flutas_batch *b, hipStream_t s) {
-  for (int st = 0; st < 4; ++st) {
-    const auto &sg = b->segs[st];
+static bool x_stage(int s) {
+  return s == BS_X_FWD || s == BS_X_BWD || s == BS_X_FWD_R || s == BS_X_BWD_ACC;
+}
+static bool resid_stage(int s) { return s == BS_RESID || s == BS_CHECK; }
+
+// Item lists, one per stage: the problems with the most points first; x
+// stages (tile, plane chunk) items of the problem's current tile size,
+// slab stages one item per x-slab, residual stages (row group, plane
+// chunk) items.  Sets the dynamic LDS and the grids, and drops the
+// captured graphs.
+static int batch_items(flutas_batch *b) {
+  const int KP = batch_resid_planes();
+  std::vector<int> order(b->nprob);
+  for (int q = 0; q < b->nprob; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const BatchProblem &px = b->probs[x], &py = b->probs[y];
+    return (long long)px.nx * px.ny * px.nz > (long long)py.nx * py.ny * py.nz;
+  });
+  size_t lds = 0;
+  for (BatchProblem &p : b->probs) {
+    p.rpi = batch_resid_rows(p.nx);
+    lds = std::max(lds, batch_item_lds(0, p.nx, b->elem, p.tj_v));
+    lds = std::max(lds, batch_item_lds(1, p.ny, b->elem, p.gp_v));
+  }
+  batch_drop_graphs(b);
+  b->lds = lds;
+  FL_TRY(batch_persistent_setup(b->elem, lds, &b->max_per_cu));
+  const int per_cu = (b->per_cu > 0) ? std::min(b->per_cu, b->max_per_cu) : b->max_per_cu;
+  int s0, s1;
+  batch_stage_range(b, 0, &s0, &s1);
+  for (int s = 0; s < kBatchStages; ++s) {
+    if (b->d_items[s]) (void)hipFree(b->d_items[s]);
+    b->d_items[s] = nullptr;
+    b->nitems[s] = 0;
+    b->grid[s] = 0;
+    if (s < s0 || s > s1) continue;
+    // per-length order: the problems by line length of the pass (nx for
+    // the x stages, ny for the slab stages), longest first, then by size
+    std::vector<int> ord = order;
+    if (!resid_stage(s))
+      std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        const int nxx = x_stage(s) ? b->probs[x].nx : b->probs[x].ny;
+        const int nyy = x_stage(s) ? b->probs[y].nx : b->probs[y].ny;
+        return nxx > nyy;
+      });
+    b->segs[s].clear();
+    std::vector<BatchWork> items;
+    for (int q : ord) {
+      const BatchProblem &p = b->probs[q];
+      const int n = resid_stage(s) ? 0 : (x_stage(s) ? p.nx : p.ny);
+      if (b->segs[s].empty() || b->segs[s].back().n != n) {
+        const size_t l = resid_stage(s) ? 0
+                         : batch_item_lds(x_stage(s) ? 0 : 1, n, b->elem,
+                                          x_stage(s) ? p.tj_v : p.gp_v);
+        b->segs[s].push_back({(int)items.size(), 0, n, l});
+      }
+      if (x_stage(s)) {
+        const int tj = batch_tile_lines(p.nx, b->elem, p.tj_v);
+        const int ntile = (p.ny + tj - 1) / tj, nkc = (p.nz + p.kxp - 1) / p.kxp;
+        for (int kc = 0; kc < nkc; ++kc)
+          for (int t = 0; t < ntile; ++t) items.push_back({s, q, t, kc});
+      } else if (resid_stage(s)) {
+        const int ng = (p.ny + p.rpi - 1) / p.rpi, nkc = (p.nz + KP - 1) / KP;
+        for (int kc = 0; kc < nkc; ++kc)
+          for (int jg = 0; jg < ng; ++jg) items.push_back({s, q, jg, kc});
+      } else {
+        for (int i = 0; i < p.nx; ++i) items.push_back({s, q, i, 0});
+      }
+      b->segs[s].back().cnt = (int)items.size() - b->segs[s].back().off;
+    }
+    for (const auto &sg : b->segs[s]) FL_TRY(batch_segment_setup(b->elem, s, sg.n, sg.lds));
+    b->nitems[s] = (int)items.size();
+    b->grid[s] = std::max(1, std::min(b->nitems[s], per_cu * b->ncu));
+    FL_HIP(hipMalloc((void **)&b->d_items[s], items.size() * sizeof(BatchWork)));
+    FL_HIP(hipMemcpy(b->d_items[s], items.data(), items.size() * sizeof(BatchWork),
+                     hipMemcpyHostToDevice));
+  }
+  return batch_upload(b);
+}
+
+// The launches of graph g on stream st: one per stage, then (refined) the
+// publishing launch.
+static int batch_enqueue(flutas_batch *b, int g, hipStream_t st) {
+  int s0, s1;
+  batch_stage_range(b, g, &s0, &s1);
+  for (int s = s0; s <= s1; ++s) {
+    BatchPassArgs A;
+    A.P = b->d_probs;
+    A.items = b->d_items[s];
+    A.nitems = b->nitems[s];
+    A.mx = b->d_mx;
+    if (b->mode == 1) {
+      FL_TRY(launch_batch_pass(b->elem, A, b->grid[s], b->lds, st));
+      continue;
+    }
+    const auto &sg = b->segs[s];
     auto launch = [&](size_t q, hipStream_t t) {
-      return launch_batch_stage(st, b->d_probs, b->d_items[st] + sg[q].off, sg[q].cnt, sg[q].lds,
-                                sg[q].n, t);
+      BatchPassArgs a = A;
+      a.items = A.items + sg[q].off;
+      a.nitems = sg[q].cnt;
+      return launch_batch_segment(b->elem, a, s, sg[q].n, sg[q].lds, t);
     };
     const size_t nseg = sg.size();
     if (nseg <= 1 || nseg > (size_t)flutas_batch::kEvents - 1) {
-      for (size_t q = 0; q < nseg; ++q) FL_TRY(launch(q, s));
+      for (size_t q = 0; q < nseg; ++q) FL_TRY(launch(q, st));
       continue;
     }
-    // fork: segment q > 0 on side stream (q-1) % kSide after the pass so far
-    hipEvent_t *e = b->ev[st];
-    FL_HIP(hipEventRecord(e[0], s));
+    // fork: segment q > 0 on side stream (q-1) % kSide after the work so
+    // far; join: the next pass starts when every segment has ended
+    hipEvent_t *e = b->ev[s];
+    FL_HIP(hipEventRecord(e[0], st));
     for (size_t q = 1; q < nseg; ++q) {
       hipStream_t t = b->side[(q - 1) % flutas_batch::kSide];
       FL_HIP(hipStreamWaitEvent(t, e[0], 0));
       FL_TRY(launch(q, t));
       FL_HIP(hipEventRecord(e[q], t));
     }
-    FL_TRY(launch(0, s));
-    // join: the next pass starts when every segment of this one has ended
-    for (size_t q = 1; q < nseg; ++q) FL_HIP(hipStreamWaitEvent(s, e[q], 0));
+    FL_TRY(launch(0, st));
+    for (size_t q = 1; q < nseg; ++q) FL_HIP(hipStreamWaitEvent(st, e[q], 0));
   }
+  if (b->precision != FLUTAS_FP64) FL_TRY(launch_batch_publish(b->d_mx, b->d_out, 3 * b->nprob, st));
+  return FLUTAS_OK;
+}
+
+// Replays graph g (captured on first use), copies the published maxima to
+// the host (refined mode) and synchronises.
+static int batch_run(flutas_batch *b, int g) {
+  if (!b->exec[g]) {
+    FL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(b, g, b->stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(FLUTAS_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    b->graph[g] = gr;
+    FL_HIP(hipGraphInstantiate(&b->exec[g], gr, nullptr, nullptr, 0));
+  }
+  FL_HIP(hipGraphLaunch(b->exec[g], b->stream));
+  if (b->precision != FLUTAS_FP64)
+    FL_HIP(hipMemcpyAsync(b->h_out, b->d_out, 3 * b->nprob * sizeof(double),
+                          hipMemcpyDeviceToHost, b->stream));
+  FL_HIP(hipStreamSynchronize(b->stream));
+  return FLUTAS_OK;
+}
+
+// The whole batched solve: graph 0, then (refined) correction rounds for
+// the problems still above their tolerance.
+static int batch_solve_all(flutas_batch *b) {
+  FL_TRY(batch_run(b, 0));
+  const int n = b->nprob;
+  b->iters.assign(n, 0);
+  b->relres.assign(n, -1.0);
+  if (b->precision == FLUTAS_FP64) return FLUTAS_OK;
+  std::vector<double> fmax(n);
+  bool more = false;
+  for (int q = 0; q < n; ++q) {
+    fmax[q] = b->h_out[3 * q];
+    const double r = b->h_out[3 * q + 2];
+    b->iters[q] = 1;
+    b->relres[q] = fmax[q] > 0.0 ? r / fmax[q] : r;
+    if (!(fmax[q] == fmax[q]) || std::isinf(fmax[q]))
+      return fail(FLUTAS_ENONFINITE, "problem " + std::to_string(q) + ": right-hand side holds NaN or Inf");
+    if (!(b->relres[q] == b->relres[q]))
+      return fail(FLUTAS_ENONFINITE, "problem " + std::to_string(q) + ": residual is NaN");
+    more |= b->relres[q] > b->tol;
+  }
+  int rc = FLUTAS_OK;
+  for (int round = 1; more && round < b->max_rounds && rc == FLUTAS_OK; ++round) {
+    for (int q = 0; q < n; ++q) b->probs[q].active = b->relres[q] > b->tol ? 1 : 0;
+    if ((rc = batch_upload(b))) break;
+    if ((rc = batch_run(b, 1))) break;
+    more = false;
+    for (int q = 0; q < n; ++q) {
+      if (!b->probs[q].active) continue;
+      const double r = b->h_out[3 * q + 2];
+      b->relres[q] = fmax[q] > 0.0 ? r / fmax[q] : r;
+      ++b->iters[q];
+      if (!(b->relres[q] == b->relres[q])) {
+        rc = fail(FLUTAS_ENONFINITE, "problem " + std::to_string(q) + ": residual is NaN");
+        break;
+      }
+      more |= b->relres[q] > b->tol;
+    }
+  }
+  // every problem active again for the next solve
+  bool changed = false;
+  for (BatchProblem &p : b->probs) {
+    changed |= p.active == 0;
+    p.active = 1;
+  }
+  if (changed) FL_TRY(batch_upload(b));
+  if (rc) return rc;
+  if (more) return fail(FLUTAS_ENOTCONV, "batched refinement did not reach the tolerance");
   return FLUTAS_OK;
 }
 
 extern "C" {
 
-int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double *dli,
-                        int nh_d, const double *const *dzci, const double *const *dzfi,
-                        flutas_batch **out) {
+int flutas_batch_create_ex(int nprob, const int *ng, const char *cbc, const double *dli,
+                           int nh_d, const double *const *dzci, const double *const *dzfi,
+                           int precision, double tol, flutas_batch **out) {
   if (!out || nprob < 1 || !ng || !cbc || !dli || !dzci || !dzfi)
     return fail(FLUTAS_EINVAL, "bad batch arguments");
+  if (precision != FLUTAS_FP64 && precision != FLUTAS_FP32_REFINE)
+    return fail(FLUTAS_EINVAL, "precision must be 64 or 32");
+  if (precision == FLUTAS_FP32_REFINE && !(tol > 0.0)) return fail(FLUTAS_EINVAL, "tol must be > 0");
   *out = nullptr;
   FL_TRY(check_device());
   flutas_batch *b = new flutas_batch;
   b->nprob = nprob;
+  b->precision = precision;
+  b->tol = tol;
+  b->elem = precision == FLUTAS_FP64 ? 8 : 4;
   int rc = FLUTAS_OK;
   const double zero_bc[6] = {0, 0, 0, 0, 0, 0};
   for (int q = 0; q < nprob && rc == FLUTAS_OK; ++q) {
@@ -1025,7 +1173,7 @@ int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double 
     }
     flutas_psolve *h = nullptr;
     rc = flutas_psolve_create(n, cbc + 6 * q, zero_bc, "ccc", dli + 3 * q, nh_d, dzci[q],
-                              dzfi[q], FLUTAS_FP64, 0.0, &h);
+                              dzfi[q], precision, precision == FLUTAS_FP64 ? 0.0 : tol, &h);
     if (rc) break;
     b->plans.push_back(h);
     if (!h->fused) {
@@ -1034,20 +1182,27 @@ int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double 
     }
     BatchProblem p{};
     p.nx = h->nx; p.ny = h->ny; p.nz = h->nz;
-    p.ncx = h->nx / 2; p.ncy = h->ny / 2;
     p.kx_f = h->kinds[0]; p.kx_b = h->kinds[1]; p.ky_f = h->kinds[2]; p.ky_b = h->kinds[3];
     p.per = (h->cbc[4] == 'P');
     p.pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || p.per;
-    p.tj_half = 0; p.gp_half = 0;
+    p.tj_v = 0; p.gp_v = 0;
+    p.kxp = 4;
+    p.active = 1;
+    std::memcpy(p.cbc, h->cbc, 6);
     p.scale = h->normfft;
+    p.dx2 = h->dli[0] * h->dli[0];
+    p.dy2 = h->dli[1] * h->dli[1];
     p.hx = h->nx + 2;
     p.hxy = (long long)(h->nx + 2) * (h->ny + 2);
-    p.W = h->W64;
+    p.W = precision == FLUTAS_FP64 ? (void *)h->W64 : (void *)h->W32;
+    p.R = h->R32;
     p.lamx = h->d_lamx; p.lamy = h->d_lamy; p.a = h->d_a; p.b = h->d_b; p.c = h->d_c;
     p.twx = h->tx.d_ptw; p.twhx = h->tx.d_twh; p.twqx = h->tx.d_twq;
     p.twy = h->ty.d_ptw; p.twhy = h->ty.d_twh; p.twqy = h->ty.d_twq;
-    // checkpoints for the halved plane group (the most groups)
-    const int gp = batch_group_planes(h->ny, 1);
+    p.ftwx = h->tx.f_ptw; p.ftwhx = h->tx.f_twh; p.ftwqx = h->tx.f_twq;
+    p.ftwy = h->ty.f_ptw; p.ftwhy = h->ty.f_twh; p.ftwqy = h->ty.f_twq;
+    // checkpoints for the quarter plane group (the most groups)
+    const int gp = batch_group_planes(h->ny, b->elem, 2);
     const size_t nck = (size_t)((h->nz + gp - 1) / gp) * (p.per ? 2 : 1);
     double *ck = nullptr;
     if (hipMalloc((void **)&ck, (size_t)h->nx * nck * h->ny * sizeof(double)) != hipSuccess) {
@@ -1057,22 +1212,38 @@ int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double 
     b->ck.push_back(ck);
     p.ck = ck;
     b->probs.push_back(p);
-    for (int st = 0; st < 4; ++st)
-      b->lds[st] = std::max(b->lds[st], batch_lds_bytes(st, (st == 0 || st == 3) ? h->nx : h->ny));
   }
-  if (rc == FLUTAS_OK && hipMalloc((void **)&b->d_probs, nprob * sizeof(BatchProblem)) != hipSuccess)
-    rc = fail(FLUTAS_ENOMEM, "batch table allocation");
-  if (rc == FLUTAS_OK) rc = batch_items(b);
-  if (rc == FLUTAS_OK) rc = batch_set_lds(b->lds);
+  auto dalloc = [&](void **ptr, size_t bytes) {
+    if (rc == FLUTAS_OK && hipMalloc(ptr, bytes) != hipSuccess)
+      rc = fail(FLUTAS_ENOMEM, "batch table allocation");
+    if (rc == FLUTAS_OK && hipMemset(*ptr, 0, bytes) != hipSuccess)
+      rc = fail(FLUTAS_EDEVICE, "hipMemset");
+  };
+  dalloc((void **)&b->d_probs, nprob * sizeof(BatchProblem));
+  dalloc((void **)&b->d_mx, 3 * (size_t)nprob * sizeof(unsigned long long));
+  dalloc((void **)&b->d_out, 3 * (size_t)nprob * sizeof(double));
+  if (rc == FLUTAS_OK &&
+      hipHostMalloc((void **)&b->h_out, 3 * (size_t)nprob * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess)
+    rc = fail(FLUTAS_ENOMEM, "pinned host allocation");
+  if (rc == FLUTAS_OK) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+      rc = fail(FLUTAS_EDEVICE, "hipGetDeviceProperties");
+    else
+      b->ncu = prop.multiProcessorCount;
+  }
   if (rc == FLUTAS_OK && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(FLUTAS_EDEVICE, "hipStreamCreateWithFlags");
   for (int t = 0; t < flutas_batch::kSide && rc == FLUTAS_OK; ++t)
     if (hipStreamCreateWithFlags(&b->side[t], hipStreamNonBlocking) != hipSuccess)
       rc = fail(FLUTAS_EDEVICE, "hipStreamCreateWithFlags");
-  for (int st = 0; st < 4 && rc == FLUTAS_OK; ++st)
+  for (int st = 0; st < kBatchStages && rc == FLUTAS_OK; ++st)
     for (int q = 0; q < flutas_batch::kEvents && rc == FLUTAS_OK; ++q)
       if (hipEventCreateWithFlags(&b->ev[st][q], hipEventDisableTiming) != hipSuccess)
         rc = fail(FLUTAS_EDEVICE, "hipEventCreateWithFlags");
+  if (rc == FLUTAS_OK) rc = batch_items(b);
   if (rc) {
     batch_release(b);
     delete b;
@@ -1082,10 +1253,17 @@ int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double 
   return FLUTAS_OK;
 }
 
+int flutas_batch_create(int nprob, const int *ng, const char *cbc, const double *dli,
+                        int nh_d, const double *const *dzci, const double *const *dzfi,
+                        flutas_batch **out) {
+  return flutas_batch_create_ex(nprob, ng, cbc, dli, nh_d, dzci, dzfi, FLUTAS_FP64, 0.0, out);
+}
+
 int flutas_batch_set_fields(flutas_batch *b, const double *const *f, double *const *p) {
   if (!b || !f || !p) return fail(FLUTAS_EINVAL, "null pointer");
   for (int q = 0; q < b->nprob; ++q) {
     if (!f[q] || !p[q]) return fail(FLUTAS_EINVAL, "null field pointer");
+    if (f[q] == p[q]) return fail(FLUTAS_EINVAL, "batched solves need f and p apart");
     const long long off = 1 + b->probs[q].hx + b->probs[q].hxy;
     b->probs[q].f = f[q] + off;
     b->probs[q].p = p[q] + off;
@@ -1100,76 +1278,144 @@ int flutas_batch_solve(flutas_batch *b) {
   if (!b->fields_set) return fail(FLUTAS_EINVAL, "flutas_batch_set_fields not called");
   // the caller's pending work on the library stream comes first
   FL_HIP(hipStreamSynchronize(current_stream()));
-  if (!b->exec) {
-    FL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = batch_enqueue(b, b->stream);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (e != hipSuccess) return fail(FLUTAS_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-    b->graph = g;
-    FL_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return batch_solve_all(b);
+}
+
+int flutas_batch_stats(const flutas_batch *b, int *iters, double *relres) {
+  if (!b) return fail(FLUTAS_EINVAL, "null batch");
+  for (int q = 0; q < b->nprob; ++q) {
+    if (iters) iters[q] = q < (int)b->iters.size() ? b->iters[q] : 0;
+    if (relres) relres[q] = q < (int)b->relres.size() ? b->relres[q] : -1.0;
   }
-  FL_HIP(hipGraphLaunch(b->exec, b->stream));
-  FL_HIP(hipStreamSynchronize(b->stream));
   return FLUTAS_OK;
 }
 
-// Autotuner: per x length, the full or halved x tile; per y length, the
-// full or halved slab plane group.  Coordinate descent over the size
-// classes present, each choice timed over `reps` graph replays.
+// Autotuner: coordinate descent over the launch shape (per-length launches
+// or persistent pass walks, the latter at its best workgroups per CU),
+// then per x length the tile variant (full, half, quarter lines), per y
+// length the plane-group variant (the z elimination's block of planes),
+// the planes of an x-tile item (1, 2, 4, 8), then the launch shape again; each choice timed over `reps` replays of
+// graph 0 (the whole solve in FP64; solve, residual, correction and check
+// in the refined mode).
 int flutas_batch_tune(flutas_batch *b, int reps) {
   if (!b || reps < 1) return fail(FLUTAS_EINVAL, "bad tune arguments");
   if (!b->fields_set) return fail(FLUTAS_EINVAL, "flutas_batch_set_fields not called");
+  FL_HIP(hipStreamSynchronize(current_stream()));
   hipEvent_t e0, e1;
   FL_HIP(hipEventCreate(&e0));
   FL_HIP(hipEventCreate(&e1));
   auto timed = [&](float *ms) -> int {
-    FL_TRY(batch_upload(b));
     FL_TRY(batch_items(b));
-    FL_TRY(flutas_batch_solve(b));  // capture and warm
+    FL_TRY(batch_run(b, 0));  // capture and warm
     FL_HIP(hipEventRecord(e0, b->stream));
-    for (int r = 0; r < reps; ++r) FL_HIP(hipGraphLaunch(b->exec, b->stream));
+    for (int r = 0; r < reps; ++r) FL_HIP(hipGraphLaunch(b->exec[0], b->stream));
     FL_HIP(hipEventRecord(e1, b->stream));
     FL_HIP(hipEventSynchronize(e1));
     FL_HIP(hipEventElapsedTime(ms, e0, e1));
     return FLUTAS_OK;
   };
   int rc = FLUTAS_OK;
+  auto tune_per_cu = [&]() -> int {
+    FL_TRY(batch_items(b));
+    const int top = b->max_per_cu;
+    float best_t = 0.f;
+    int best = top;
+    for (int pc = 1; pc <= top; ++pc) {
+      b->per_cu = pc;
+      float t = 0.f;
+      FL_TRY(timed(&t));
+      if (pc == 1 || t < best_t) { best_t = t; best = pc; }
+    }
+    b->per_cu = best;
+    return FLUTAS_OK;
+  };
+  // launch shape: per-length launches or persistent pass walks (each at
+  // its own best occupancy)
+  auto tune_mode = [&]() -> int {
+    float t[2] = {0.f, 0.f};
+    int pc1 = b->per_cu;
+    for (int m = 0; m < 2; ++m) {
+      b->mode = m;
+      if (m == 1) {
+        FL_TRY(tune_per_cu());
+        pc1 = b->per_cu;
+      }
+      FL_TRY(timed(&t[m]));
+    }
+    b->mode = t[1] < t[0] ? 1 : 0;
+    b->per_cu = pc1;
+    return FLUTAS_OK;
+  };
+  rc = tune_mode();
   for (int axis = 0; axis < 2 && rc == FLUTAS_OK; ++axis) {
     std::vector<int> classes;
     for (const BatchProblem &p : b->probs) classes.push_back(axis == 0 ? p.nx : p.ny);
     std::sort(classes.begin(), classes.end());
     classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
     for (int n : classes) {
-      float t[2] = {0.f, 0.f};
-      for (int half = 0; half < 2 && rc == FLUTAS_OK; ++half) {
+      float t[3] = {0.f, 0.f, 0.f};
+      for (int v = 0; v < 3 && rc == FLUTAS_OK; ++v) {
         for (BatchProblem &p : b->probs)
-          if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_half : p.gp_half) = half;
-        rc = timed(&t[half]);
+          if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_v : p.gp_v) = v;
+        rc = timed(&t[v]);
       }
       if (rc) break;
-      const int best = t[1] < t[0] ? 1 : 0;
+      int best = 0;
+      for (int v = 1; v < 3; ++v)
+        if (t[v] < t[best]) best = v;
       for (BatchProblem &p : b->probs)
-        if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_half : p.gp_half) = best;
+        if ((axis == 0 ? p.nx : p.ny) == n) (axis == 0 ? p.tj_v : p.gp_v) = best;
     }
   }
+  // planes per x-tile item (fewer: more workgroups; more: the twiddles
+  // staged once for more planes)
+  if (rc == FLUTAS_OK) {
+    const int opts[4] = {1, 2, 4, 8};
+    float t[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int o = 0; o < 4 && rc == FLUTAS_OK; ++o) {
+      for (BatchProblem &p : b->probs) p.kxp = opts[o];
+      rc = timed(&t[o]);
+    }
+    int best = 0;
+    for (int o = 1; o < 4; ++o)
+      if (t[o] < t[best]) best = o;
+    for (BatchProblem &p : b->probs) p.kxp = opts[best];
+  }
+  if (rc == FLUTAS_OK) rc = tune_mode();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  FL_TRY(batch_upload(b));
   return batch_items(b);
 }
 
-int flutas_batch_tuning(const flutas_batch *b, int *tile_half, int *group_half) {
+int flutas_batch_tuning(const flutas_batch *b, int *tile_variant, int *group_variant) {
   if (!b) return fail(FLUTAS_EINVAL, "null batch");
   for (int q = 0; q < b->nprob; ++q) {
-    if (tile_half) tile_half[q] = b->probs[q].tj_half;
-    if (group_half) group_half[q] = b->probs[q].gp_half;
+    if (tile_variant) tile_variant[q] = b->probs[q].tj_v;
+    if (group_variant) group_variant[q] = b->probs[q].gp_v;
   }
+  return FLUTAS_OK;
+}
+
+int flutas_batch_launch_info(const flutas_batch *b, int *workgroups, int *items,
+                             int *per_cu, size_t *lds_bytes) {
+  if (!b) return fail(FLUTAS_EINVAL, "null batch");
+  int wg = 0, it = 0;
+  for (int s = 0; s < kBatchStages; ++s) {
+    wg = std::max(wg, b->grid[s]);
+    it += b->nitems[s];
+  }
+  if (workgroups) *workgroups = wg;
+  if (items) *items = it;
+  if (per_cu) *per_cu = b->per_cu > 0 ? std::min(b->per_cu, b->max_per_cu) : b->max_per_cu;
+  if (lds_bytes) *lds_bytes = b->lds;
+  return FLUTAS_OK;
+}
+
+int flutas_batch_launch_shape(const flutas_batch *b, int *shape, int *x_item_planes) {
+  if (!b) return fail(FLUTAS_EINVAL, "null batch");
+  if (shape) *shape = b->mode;
+  if (x_item_planes) *x_item_planes = b->probs.empty() ? 0 : b->probs[0].kxp;
   return FLUTAS_OK;
 }
 

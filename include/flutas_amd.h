@@ -159,30 +159,61 @@ int flutas_fft_guru(int kind, int n, long long stride, int howmany1,
 /* ---- batched solves (no counterpart in the reference, which solves one
  *      problem per call; for many independent small solves) ------------- */
 typedef struct flutas_batch flutas_batch;
-/* nprob independent FP64 problems, problem q described by ng[3q..3q+2],
+/* nprob independent problems, problem q described by ng[3q..3q+2],
  * cbc[6q..6q+5], dli[3q..3q+2] and the host metric arrays dzci[q],
  * dzfi[q] (1-nh_d : nz+nh_d), as for flutas_psolve_create with zero BC
  * values and c_or_f "ccc".  Every nx and ny in {32, 48, 64, 96, 128, 192,
- * 256} and nz >= 3, else EUNSUPPORTED. */
+ * 256} and nz >= 3, else EUNSUPPORTED.  precision FLUTAS_FP64 or
+ * FLUTAS_FP32_REFINE (FP32 transforms, FP64 residuals and z elimination,
+ * each problem refined to a relative residual <= tol). */
+int flutas_batch_create_ex(int nprob, const int *ng, const char *cbc,
+                           const double *dli, int nh_d,
+                           const double *const *dzci,
+                           const double *const *dzfi, int precision,
+                           double tol, flutas_batch **out);
+/* flutas_batch_create_ex with FLUTAS_FP64. */
 int flutas_batch_create(int nprob, const int *ng, const char *cbc,
                         const double *dli, int nh_d,
                         const double *const *dzci, const double *const *dzfi,
                         flutas_batch **out);
 /* The halo-1 device fields of every problem: f[q] is read (interior), the
- * interior of p[q] receives the solution.  May be called again with other
- * fields; the captured graph is kept. */
+ * interior of p[q] receives the solution (f[q] != p[q]).  May be called
+ * again with other fields; the captured graphs are kept. */
 int flutas_batch_set_fields(flutas_batch *b, const double *const *f,
                             double *const *p);
-/* Solves every problem.  Each of the four passes is one launch over all
- * problems; the launches are captured into a hipGraph on the first call
- * and replayed after.  Returns when every solution is in place. */
+/* Solves every problem.  Each pass of the pipeline (x transform, y
+ * transform + forward elimination, back substitution + inverse y
+ * transform, inverse x transform; in the refined mode also the FP64
+ * residual, the correction solve and the residual check) is one persistent
+ * launch whose workgroups walk a list of (problem, tile or slab) items over
+ * every problem and line length; the launches of a solve are captured into
+ * a hipGraph on the first call and replayed.  Problems still above tol get
+ * further correction rounds (one graph replay each, at most 7).  Returns
+ * when every solution is in place; FLUTAS_ENOTCONV if a problem stays above
+ * tol, FLUTAS_ENONFINITE for a NaN or Inf. */
 int flutas_batch_solve(flutas_batch *b);
-/* Autotuner: for each x length the full or halved x tile, for each y
- * length the full or halved slab plane group, chosen by timing `reps`
- * replays of the whole batch per choice (the fields must be set).
- * flutas_batch_tuning reports the choices per problem (1 = halved). */
+/* Per problem: correction rounds of the last solve (0 in FP64 mode) and
+ * its relative residual max|f - L p| / max|f| (-1 in FP64 mode). */
+int flutas_batch_stats(const flutas_batch *b, int *iters, double *relres);
+/* Autotuner (the fields must be set): coordinate descent over the launch
+ * shape of the passes (0: one launch per pass and line length, one
+ * workgroup per item, run side by side; 1: one persistent launch per pass
+ * whose workgroups walk every item, at its best workgroups per CU), the
+ * x-tile variant of each x length and the plane-group variant
+ * (z-elimination block) of each y length (0 full, 1 half, 2 quarter of the
+ * LDS-sized default) and the planes of an x-tile work item (1, 2, 4, 8),
+ * each choice timed over `reps` replays of the whole solve.
+ * flutas_batch_tuning reports the variants per problem,
+ * flutas_batch_launch_shape the shape and the planes per x-tile item,
+ * flutas_batch_launch_info the persistent launch (workgroups, items over
+ * all passes, workgroups per CU, dynamic LDS bytes). */
 int flutas_batch_tune(flutas_batch *b, int reps);
-int flutas_batch_tuning(const flutas_batch *b, int *tile_half, int *group_half);
+int flutas_batch_tuning(const flutas_batch *b, int *tile_variant,
+                        int *group_variant);
+int flutas_batch_launch_info(const flutas_batch *b, int *workgroups,
+                             int *items, int *per_cu, size_t *lds_bytes);
+int flutas_batch_launch_shape(const flutas_batch *b, int *shape,
+                              int *x_item_planes);
 int flutas_batch_destroy(flutas_batch *b);
 
 /* ---- restart files in the reference's formats ------------------------- */

@@ -10,7 +10,8 @@
       (PP/PP/NN, tanh stretching 1.5): FP64 and FP32 + refinement against
       the oracle's solve of the same right-hand side.
   batch_problems()               the seeded 64-problem uneven batch: every
-      problem against the oracle.
+      problem against the oracle, plain FP64 (<= 1e-12) and FP32 +
+      refinement (relative residual <= 1e-10, <= 1e-8 gauge-free).
   neumann_1024x1024x512          NN/NN/NN (4 GiB per FP64 field): too
       large for the oracle in a test, checked through size-independent
       properties: the relative residual of the FP64 solve, the refined
@@ -233,4 +234,43 @@ def test_batch_problems_against_oracle(api, oracle):
         worst = max(worst, err)
         assert err <= 1e-12, (q, case.ng, case.cbc, case.alpha, err)
     print(f"worst relative difference vs oracle: {worst:.2e}", flush=True)
+    batch.destroy()
+
+
+def test_batch_problems_refined_against_oracle(api, oracle):
+    """The seeded uneven batch in the refined mode (FP32 transforms, FP64
+    residuals and z elimination, one persistent launch per round): every
+    problem reaches a relative residual <= 1e-10 (reported by the batch and
+    measured again with a single FP64 plan) and agrees with the oracle to
+    1e-8 gauge-free."""
+    from flutas_amd import workloads as wl
+    cases = wl.batch_problems()
+    problems, fs, ps, rhs = [], [], [], []
+    for case in cases:
+        dzc, dzf, dzci, dzfi = wl.z_grid(oracle, case)
+        f = wl.rhs_normal(case, dzf)
+        problems.append((case.ng, case.dli, dzci, dzfi, case.cbc))
+        rhs.append(f)
+        fs.append(api.DeviceArray.from_host(f))
+        ps.append(api.DeviceArray(f.shape))
+    batch = api.BatchPlan(problems, precision=api.FP32_REFINE, tol=1e-10)
+    batch.set_fields(fs, ps)
+    batch.solve()
+    iters, rel = batch.stats()
+    print(f"refined batch: rounds {sorted(set(iters))}, worst relres {max(rel):.2e}", flush=True)
+    worst = 0.0
+    for q, (case, (ng, dli, dzci, dzfi, cbc), f, fd, p) in enumerate(
+            zip(cases, problems, rhs, fs, ps)):
+        assert 1 <= iters[q] <= 4 and 0.0 <= rel[q] <= 1e-10, (q, iters[q], rel[q])
+        plan = api.PoissonPlan(ng, dli, dzci, dzfi, cbc)
+        check = plan.residual(fd, p)
+        plan.destroy()
+        assert check <= 1e-10, (q, case.ng, case.cbc, check)
+        s = oracle.Solver(ng, dli, dzci, dzfi, cbc)
+        ref = f.copy(order="F")
+        s.solve(ref)
+        err = relmax(gauge_free(interior(p.to_host())), gauge_free(interior(ref)))
+        worst = max(worst, err)
+        assert err <= 1e-8, (q, case.ng, case.cbc, case.alpha, err)
+    print(f"worst gauge-free difference vs oracle: {worst:.2e}", flush=True)
     batch.destroy()

@@ -404,6 +404,15 @@ def chkdiv(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w):
     return tot.value, mx.value
 
 
+def chkdiv_noalloc(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w):
+    """chkdiv through the allocation-free entry point (fixed device scratch)."""
+    tot, mx = C.c_double(), C.c_double()
+    _check(lib.flutas_chkdiv_noalloc(nx, ny, nz, C.c_double(dxi), C.c_double(dyi),
+                                     C.c_double(dzi), nh_d, nh_u, _dptr(dzfi), _dptr(u),
+                                     _dptr(v), _dptr(w), C.byref(tot), C.byref(mx)))
+    return tot.value, mx.value
+
+
 def boundp(cbc, n, bc, dl, dzc_0, dzc_n, p):
     ngc = (C.c_int * 3)(*[int(v) for v in n])
     bcv = (C.c_double * 6)(*[float(v) for v in np.asarray(bc, np.float64).ravel()])

@@ -7,7 +7,9 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 
 namespace flutas {
 
@@ -294,6 +296,40 @@ static int chkdiv_enqueue(int nx, int ny, int nz, int nh_d, int nh_u, double dxi
   hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(kRedThreads), 0, s, work, nb, work + 2 * nb);
   FL_HIP(hipGetLastError());
   FL_HIP(hipMemcpyAsync(res_host, work + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  return FLUTAS_OK;
+}
+
+// ---- scratch of flutas_chkdiv_noalloc -------------------------------------
+// The block partials and the two results of one chkdiv live in this
+// file-scope device array: each GPU the library runs on holds its own copy
+// (32 KiB) until the process ends.  In host code `scratch` is only a
+// stand-in for the symbol; kernels and copies get the device address that
+// hipGetSymbolAddress returns for the current device, resolved once per
+// device on its first call and kept in chkdiv_scratch_ptr.
+constexpr int kNoAllocDoubles = 2 * kChkdivBlocks + 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::atomic<double *> g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int chkdiv_scratch_ptr(double **ptr) {
+  int dev = 0;
+  FL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(FLUTAS_EDEVICE, "device index beyond the scratch table");
+  double *p = g_scratch_dev[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    p = g_scratch_dev[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void *addr = nullptr;
+      FL_HIP(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)));
+      if (!addr) return fail(FLUTAS_EDEVICE, "no device address for the chkdiv scratch");
+      p = static_cast<double *>(addr);
+      g_scratch_dev[dev].store(p, std::memory_order_release);
+    }
+  }
+  *ptr = p;
   return FLUTAS_OK;
 }
 
@@ -855,6 +891,28 @@ int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
   FL_HIP(hipFreeAsync(work, s));
   FL_HIP(hipStreamSynchronize(s));
   if (rc) return rc;
+  *divtot = res[0];
+  *divmax = res[1];
+  return FLUTAS_OK;
+}
+
+int flutas_chkdiv_noalloc(int nx, int ny, int nz, double dxi, double dyi, double dzi,
+                          int nh_d, int nh_u, const double *dzfi, const double *u,
+                          const double *v, const double *w, double *divtot,
+                          double *divmax) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!dzfi || !u || !v || !w || !divtot || !divmax) return fail(FLUTAS_EINVAL, "null pointer");
+  const int nb = chkdiv_blocks((long long)nx * ny * nz);
+  if (2 * nb + 2 > kNoAllocDoubles)
+    return fail(FLUTAS_EINVAL, "chkdiv partials exceed the fixed scratch of " +
+                                   std::to_string(kNoAllocDoubles) + " doubles");
+  double *work = nullptr;
+  FL_TRY(chkdiv_scratch_ptr(&work));
+  hipStream_t s = current_stream();
+  double res[2] = {0.0, 0.0};
+  FL_TRY(chkdiv_enqueue(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, work, res, s));
+  FL_HIP(hipStreamSynchronize(s));
   *divtot = res[0];
   *divmax = res[1];
   return FLUTAS_OK;

@@ -315,6 +315,20 @@ int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
                   const double *v, const double *w, double *divtot,
                   double *divmax);
 
+/* flutas_chkdiv without any device allocation.  No counterpart in the
+ * reference: it serves callers that cannot allocate (code under HIP graph
+ * capture, tight time loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results (bit for bit) as flutas_chkdiv; the block
+ * partials live in a fixed per-GPU device array of 2*2048+2 doubles
+ * (32 KiB) that the library holds until the process ends.  The kernel
+ * strides over the grid, so every field size fits those 2048 partials;
+ * a partial count beyond them is rejected with FLUTAS_EINVAL.  Calls that
+ * overlap on one GPU share that array and race: serialise them. */
+int flutas_chkdiv_noalloc(int nx, int ny, int nz, double dxi, double dyi,
+                          double dzi, int nh_d, int nh_u, const double *dzfi,
+                          const double *u, const double *v, const double *w,
+                          double *divtot, double *divmax);
+
 /* Fused prologue of the projection step: fillps then updt_rhs_b
  * (src/fillps.f90:16-26 with src/bound.f90:829-944, called back to back in
  * main__single_phase.f90:707-709) in one pass; bit-identical to the two

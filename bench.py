@@ -114,8 +114,9 @@ STEP_SIDE_BYTES = {"reference_order": 128, "fused": 104}
 def step_bench(api, case, dzc, dzf, dzci, dzfi, f_host, steps, warmup):
     """One projection step around the FP64 solve at the bench size, in the
     reference's call order (main__single_phase.f90:705-747, bounduvw after
-    correc) and in the fused order with swapped pressure buffers; ms per
-    step from HIP events.  Velocity BCs follow the pressure pattern:
+    correc) and in the fused order with swapped pressure buffers (fillps_rhs,
+    solve, correc_update_bc, bounduvw), eagerly and as one replayed graph;
+    ms per step from HIP events.  Velocity BCs follow the pressure pattern:
     periodic stays periodic, a pressure N wall is a no-slip (D, 0) wall."""
     nx, ny, nz = case.ng
     D = api.DeviceArray.from_host
@@ -146,21 +147,26 @@ def step_bench(api, case, dzc, dzf, dzci, dzfi, f_host, steps, warmup):
         api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
 
     def fused():
+        # boundp(phi), correc, p = pold + phi and boundp(p) in one pass
         api.fillps_rhs(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzfi_d, 1 / dt, 1.0,
                        u, v, w, "ccc", case.cbc, rx, ry, rz, scratch)
         plan.solve_async(scratch)
-        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], scratch)
-        api.correc_update(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0,
-                          scratch, p, p, u, v, w)
+        api.correc_update_bc(nx, ny, nz, 1, 1, dli[0], dli[1], dli[2], dzci_d, dt, 1.0,
+                             scratch, p, p, u, v, w, case.cbc, zero_bc, dl, dzc[0], dzc[nz])
         bounduvw()
-        api.boundp(case.cbc, case.ng, zero_bc, dl, dzc[0], dzc[nz], p)
 
     t_ref = time_steps(reference_order, steps, warmup, api)
     t_fused = time_steps(fused, steps, warmup, api)
     t_solve = time_steps(lambda: plan.solve_async(scratch), steps, warmup, api)
+    # the fused step recorded once (flutas_graph_*) and replayed
+    with api.Graph() as graph:
+        fused()
+    t_graph = time_steps(graph.launch, steps, warmup, api)
+    graph.destroy()
     plan.destroy()
     npts = case.npoints
     out = {"reference_order_ms": t_ref["mean"], "fused_ms": t_fused["mean"],
+           "fused_graph_ms": t_graph["mean"],
            "solve_ms": t_solve["mean"], "side_bytes_per_point": STEP_SIDE_BYTES}
     for key, t in (("reference_order", t_ref), ("fused", t_fused)):
         side = t["mean"] - t_solve["mean"]
@@ -363,7 +369,8 @@ def main():
                                                max(3, args.steps // 2), args.warmup)
         st = result["projection_step"]
         log(f"bench: step {st['reference_order_ms']:.3f} ms in the reference order, "
-            f"{st['fused_ms']:.3f} ms fused (solve {st['solve_ms']:.3f} ms; side passes "
+            f"{st['fused_ms']:.3f} ms fused, {st['fused_graph_ms']:.3f} ms as one graph "
+            f"(solve {st['solve_ms']:.3f} ms; side passes "
             f"{st['reference_order_side_GBps']:.0f} / {st['fused_side_GBps']:.0f} GB/s)")
     if not args.no_batch:
         log("bench: uneven batch of 64 problems ...")

@@ -246,6 +246,45 @@ class PoissonPlan:
             pass
 
 
+class Graph:
+    """A recorded sequence of asynchronous library calls (flutas_graph_*):
+
+        with api.Graph() as g:       # calls inside are recorded, not run
+            plan.solve_async(p)
+            api.boundp(...)
+        g.launch()                   # replays them as one launch
+    """
+
+    def __init__(self):
+        self._h = None
+
+    def __enter__(self):
+        _check(lib.flutas_graph_begin())
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = _VP()
+        rc = lib.flutas_graph_end(C.byref(h))
+        if exc_type is None:
+            _check(rc)
+            self._h = h.value
+        return False
+
+    def launch(self):
+        _check(lib.flutas_graph_launch(_VP(self._h)))
+
+    def destroy(self):
+        if self._h:
+            lib.flutas_graph_destroy(_VP(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class BatchPlan:
     """Many independent problems solved together (flutas_batch_*): one
     persistent launch over a work queue of every pass of every problem,
@@ -477,6 +516,22 @@ def correc_update(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, phi, po
                                     C.c_double(dt), C.c_double(rho0), _dptr(phi),
                                     _dptr(pold), _dptr(pnew), _dptr(u), _dptr(v),
                                     _dptr(w), rp, 1 if const_coeff else 0))
+
+
+def correc_update_bc(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, phi, pold,
+                     pnew, u, v, w, cbc, bc, dl, dzc_0, dzc_n, rho=None, const_coeff=True):
+    """boundp(phi), correc_update and boundp(pnew) in one pass
+    (flutas_correc_update_bc); phi's halo is neither read nor written."""
+    rp = _dptr(rho) if rho is not None else _D()
+    bcv = (C.c_double * 6)(*[float(x) for x in np.asarray(bc, np.float64).ravel()])
+    dlv = (C.c_double * 3)(*[float(x) for x in dl])
+    _check(lib.flutas_correc_update_bc(nx, ny, nz, nh_d, nh_u, C.c_double(dxi),
+                                       C.c_double(dyi), C.c_double(dzi), _dptr(dzci),
+                                       C.c_double(dt), C.c_double(rho0), _dptr(phi),
+                                       _dptr(pold), _dptr(pnew), _dptr(u), _dptr(v),
+                                       _dptr(w), rp, 1 if const_coeff else 0,
+                                       _chars(_bc6(cbc), 6), bcv, dlv, C.c_double(dzc_0),
+                                       C.c_double(dzc_n)))
 
 
 def copy_interior(nx, ny, nz, src, dst):

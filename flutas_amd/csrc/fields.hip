@@ -168,39 +168,128 @@ __global__ void __launch_bounds__(kRowThreads)
 }
 
 // ---- fused epilogue: correc, then pnew = pold + phi ---------------------
-// phi is the solver's increment (halos set by boundp) and is only read, so
-// the pass has no neighbour hazards; pnew may be pold (in place) but not
-// phi.  Same arithmetic as correc and the increment update.
+// phi is the solver's increment and is only read, so the pass has no
+// neighbour hazards; pnew may be pold (in place) but not phi.  Same
+// arithmetic as correc and the increment update.
+//
+// BC = true also stands in for the two boundp calls around it
+// (main__single_phase.f90:727-747: boundp(phi) before correc, boundp(p)
+// after the update): the halo values of phi that correc reads (the upper
+// faces i = nx+1, j = ny+1, k = nz+1 at interior tangential positions,
+// each written by one face of set_bc) are computed from phi's interior
+// with set_bc's rule instead of read, and every halo point of pnew is
+// written by the thread that owns its interior source point, as the
+// composition of set_bc's face rules in boundp's face order (periodic y,
+// periodic z, x, then the other y and z faces), which is the value the
+// sequence of face passes leaves there.  Bit for bit the three calls.
+struct HaloBc {
+  char t[3][2];      // BC type per direction and side
+  double val[3][2];  // BC value
+  double dr[3][2];   // spacing of the N rule (dl in x and y, dzc(0)/dzc(nz) in z)
+  int rank[3];       // position of the direction's faces in boundp's order
+};
+
+// set_bc's rule for one face (k_set_bc): the ghost value from the value
+// one cell inside.
+__device__ __forceinline__ double face_rule(char t, int side, double value, double dr, double v) {
+  if (t == 'P') return v;
+  double factor = value, sgn = 0.0;
+  if (t == 'D') { factor = 2.0 * factor; sgn = -1.0; }
+  if (t == 'N') { factor = (side == 0) ? -dr * factor : dr * factor; sgn = 1.0; }
+  return factor + sgn * v;
+}
+
+template <bool BC>
 __global__ void __launch_bounds__(kRowThreads)
     k_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
                     double factorj, const double *__restrict__ dzci, double dt,
                     double rho0i, const double *__restrict__ phi,
                     const double *pold, double *pnew, double *__restrict__ u,
                     double *__restrict__ v, double *__restrict__ w,
-                    const double *__restrict__ rho, int cc) {
+                    const double *__restrict__ rho, int cc, HaloBc hb) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > nx) return;
+  const long long sy = nx + 2, sz = (long long)(nx + 2) * (ny + 2);
   const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
-  const long long ip = c1 + 1, jp = c1 + (nx + 2),
-                  kp = c1 + (long long)(nx + 2) * (ny + 2);
+  const long long ip = c1 + 1, jp = c1 + sy, kp = c1 + sz;
   const double pc = phi[c1];
+  double phx, phy, phz;
+  if constexpr (BC) {
+    // upper-face ghosts of phi: periodic copies of the first interior
+    // plane, else set_bc's rule on the last one
+    phx = (i < nx) ? phi[ip]
+          : (hb.t[0][1] == 'P' ? phi[c1 - (nx - 1)]
+                               : face_rule(hb.t[0][1], 1, hb.val[0][1], hb.dr[0][1], pc));
+    phy = (j < ny) ? phi[jp]
+          : (hb.t[1][1] == 'P' ? phi[c1 - sy * (ny - 1)]
+                               : face_rule(hb.t[1][1], 1, hb.val[1][1], hb.dr[1][1], pc));
+    phz = (k < nz) ? phi[kp]
+          : (hb.t[2][1] == 'P' ? phi[c1 - sz * (nz - 1)]
+                               : face_rule(hb.t[2][1], 1, hb.val[2][1], hb.dr[2][1], pc));
+  } else {
+    phx = phi[ip];
+    phy = phi[jp];
+    phz = phi[kp];
+  }
   const double fz = dt * dzci[k + nh_d - 1];
   const double po = pold[c1];
   if (cc) {
-    u[cu] = u[cu] - factori * (phi[ip] - pc) * rho0i;
-    v[cu] = v[cu] - factorj * (phi[jp] - pc) * rho0i;
-    w[cu] = w[cu] - fz * (phi[kp] - pc) * rho0i;
+    u[cu] = u[cu] - factori * (phx - pc) * rho0i;
+    v[cu] = v[cu] - factorj * (phy - pc) * rho0i;
+    w[cu] = w[cu] - fz * (phz - pc) * rho0i;
   } else {
     const double rc = rho[c1];
     const double rhox = 0.5 * (rho[ip] + rc), rhoy = 0.5 * (rho[jp] + rc),
                  rhoz = 0.5 * (rho[kp] + rc);
-    u[cu] = u[cu] - factori * (phi[ip] - pc) / rhox;
-    v[cu] = v[cu] - factorj * (phi[jp] - pc) / rhoy;
-    w[cu] = w[cu] - fz * (phi[kp] - pc) / rhoz;
+    u[cu] = u[cu] - factori * (phx - pc) / rhox;
+    v[cu] = v[cu] - factorj * (phy - pc) / rhoy;
+    w[cu] = w[cu] - fz * (phz - pc) / rhoz;
   }
-  pnew[c1] = po + pc;
+  const double pv = po + pc;
+  pnew[c1] = pv;
+  if constexpr (BC) {
+    // the halo points whose source is this interior point: per direction,
+    // P: halo 0 from the last plane, halo n+1 from the first; D, N: halo
+    // 0 from the first plane, halo n+1 from the last
+    const int c[3] = {i, j, k}, n[3] = {nx, ny, nz};
+    const long long st[3] = {1, sy, sz};
+    int hc[3], side[3];
+    bool has[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      has[d] = false;
+      hc[d] = c[d];
+      side[d] = 0;
+      const bool per = hb.t[d][0] == 'P';
+      if (c[d] == 1) { has[d] = true; side[d] = per ? 1 : 0; hc[d] = per ? n[d] + 1 : 0; }
+      else if (c[d] == n[d]) { has[d] = true; side[d] = per ? 0 : 1; hc[d] = per ? 0 : n[d] + 1; }
+    }
+    if (has[0] || has[1] || has[2]) {
+      for (int mask = 1; mask < 8; ++mask) {
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if ((mask >> d) & 1) ok = ok && has[d];
+        if (!ok) continue;
+        // apply the face rules of the masked directions in boundp's order
+        double val = pv;
+        for (int r = 0; r < 5; ++r) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            if (!((mask >> d) & 1) || hb.rank[d] != r) continue;
+            const int sd = side[d];
+            val = face_rule(hb.t[d][sd], sd, hb.val[d][sd], hb.dr[d][sd], val);
+          }
+        }
+        long long off = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) off += (long long)(((mask >> d) & 1) ? hc[d] : c[d]) * st[d];
+        pnew[off] = val;
+      }
+    }
+  }
 }
 
 // ---- chkdiv (src/chkdiv.f90:38-60): per-block partial sum and max -------
@@ -1169,9 +1258,52 @@ int flutas_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
   if (pnew == phi) return fail(FLUTAS_EINVAL, "pnew must not be the increment array phi");
   const double rho0i = 1.0 / rho0, factori = dt * dxi, factorj = dt * dyi;
   const int bx = xblock(nx);
-  hipLaunchKernelGGL(k_correc_update, grid3(nx, ny, nz, bx), dim3(bx), 0, current_stream(), nx,
-                     ny, nz, nh_d, nh_u, factori, factorj, dzci, dt, rho0i, phi, pold, pnew, u,
-                     v, w, rho, const_coeff ? 1 : 0);
+  hipLaunchKernelGGL(k_correc_update<false>, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, nh_d, nh_u, factori, factorj, dzci, dt, rho0i,
+                     phi, pold, pnew, u, v, w, rho, const_coeff ? 1 : 0, HaloBc{});
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+int flutas_correc_update_bc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                            double dyi, double dzi, const double *dzci, double dt,
+                            double rho0, const double *phi, const double *pold,
+                            double *pnew, double *u, double *v, double *w,
+                            const double *rho, int const_coeff, const char cbc[6],
+                            const double bc[6], const double dl[3], double dzc_0,
+                            double dzc_n) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
+  if (!dzci || !phi || !pold || !pnew || !u || !v || !w || (!const_coeff && !rho) || !cbc ||
+      !bc || !dl)
+    return fail(FLUTAS_EINVAL, "null pointer");
+  if (pnew == phi) return fail(FLUTAS_EINVAL, "pnew must not be the increment array phi");
+  for (int q = 0; q < 6; ++q)
+    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N')
+      return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+  for (int d = 0; d < 3; ++d)
+    if ((cbc[2 * d] == 'P') != (cbc[2 * d + 1] == 'P'))
+      return fail(FLUTAS_EINVAL, "periodic BCs must be paired");
+  HaloBc hb{};
+  for (int d = 0; d < 3; ++d)
+    for (int sd = 0; sd < 2; ++sd) {
+      hb.t[d][sd] = cbc[2 * d + sd];
+      hb.val[d][sd] = bc[2 * d + sd];
+    }
+  hb.dr[0][0] = hb.dr[0][1] = dl[0];
+  hb.dr[1][0] = hb.dr[1][1] = dl[1];
+  hb.dr[2][0] = dzc_0;
+  hb.dr[2][1] = dzc_n;
+  // boundp's face order: periodic y, periodic z, x, other y, other z
+  hb.rank[0] = 2;
+  hb.rank[1] = cbc[2] == 'P' ? 0 : 3;
+  hb.rank[2] = cbc[4] == 'P' ? 1 : 4;
+  const double rho0i = 1.0 / rho0, factori = dt * dxi, factorj = dt * dyi;
+  const int bx = xblock(nx);
+  hipLaunchKernelGGL(k_correc_update<true>, grid3(nx, ny, nz, bx), dim3(bx), 0,
+                     current_stream(), nx, ny, nz, nh_d, nh_u, factori, factorj, dzci, dt, rho0i,
+                     phi, pold, pnew, u, v, w, rho, const_coeff ? 1 : 0, hb);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }

@@ -7,6 +7,26 @@ namespace flutas {
 
 using tile::cx;
 
+// Tile of an x-pass workgroup (one-dimensional grid over ntile tiles per
+// plane times the planes).  Workgroups are dealt to the 8 XCDs round robin,
+// so the ids a and a + 8 run on one XCD at about the same time; they get
+// the neighbouring tiles 2m and 2m + 1 of one plane.  An FP32 tile of 16
+// lines covers 64 bytes of each transposed row of the work field, and its
+// neighbour the other 64: on one XCD the 128-byte line is fetched (or
+// written back) once instead of once per XCD.
+struct XTile {
+  int j0t, k;  // tile index within the plane, plane
+};
+__device__ __forceinline__ XTile x_tile_of(int ntile) {
+  const int id = blockIdx.x, total = gridDim.x;
+  int L = id;
+  if ((ntile & 1) == 0 && id < (total & ~15)) {
+    const int r = id & 15;
+    L = (id & ~15) + 2 * (r & 7) + (r >> 3);
+  }
+  return {L % ntile, L / ntile};
+}
+
 template <typename T, typename Tin, int NC, int TJ, int NT>
 __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
     k_x_fwd(const Tin *__restrict__ in, long long is1, long long is2,
@@ -15,10 +35,10 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
             double *__restrict__ absmax_part, int *nf_flag, int nf_seq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
-  double *slot = absmax_part ? absmax_part + blockIdx.x + (long long)gridDim.x * blockIdx.y
-                             : nullptr;
+  const XTile xt = x_tile_of((ny + TJ - 1) / TJ);
+  double *slot = absmax_part ? absmax_part + blockIdx.x : nullptr;
   x_fwd_tile<T, Tin, NC, TJ, NT>(in, is1, is2, Wt, ws, ny, kind, tw, twh, twq, slot, sm, stw,
-                                 blockIdx.x * TJ, blockIdx.y, nf_flag, nf_seq);
+                                 xt.j0t * TJ, xt.k, nf_flag, nf_seq);
 }
 
 template <typename T, typename Tout, int NC, int TJ, int NT, bool ACC>
@@ -28,8 +48,9 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
             const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  const XTile xt = x_tile_of((ny + TJ - 1) / TJ);
   x_bwd_tile<T, Tout, NC, TJ, NT, ACC>(Wt, ws, out, os1, os2, ny, kind, scale, tw, twh, twq, sm,
-                                       stw, blockIdx.x * TJ, blockIdx.y);
+                                       stw, xt.j0t * TJ, xt.k);
 }
 
 template <typename T, int NC, int GP, int NT, bool PER>
@@ -140,7 +161,7 @@ int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
-    dim3 grid((ny + TJ - 1) / TJ, nz);
+    dim3 grid(((ny + TJ - 1) / TJ) * nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
                        is2, Wt, ws, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part, nf_flag,
                        nf_seq);
@@ -157,7 +178,7 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
-    dim3 grid((ny + TJ - 1) / TJ, nz);
+    dim3 grid(((ny + TJ - 1) / TJ) * nz);
     if (accumulate)
       hipLaunchKernelGGL((k_x_bwd<T, Tout, NC, TJ, NT, true>), grid, dim3(NT), 0, s, Wt,
                          ws, out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);

@@ -127,8 +127,11 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 }
 
 // Minimum waves per SIMD of a workgroup size: two 512-thread workgroups
-// per CU (4 waves per SIMD, at most 128 VGPRs each), the most that the LDS
-// of two tiles or slabs allows.
+// per CU (4 waves per SIMD, at most 128 VGPRs each).  For 512-point lines
+// the LDS of two tiles or slabs (with their staged twiddles) fits a CU;
+// a 1024-point FP64 tile or slab group (8 lines or 2 x 4 planes of 1026
+// doubles, 65.7 KB, plus 25.6 KB of staged twiddles) does not, so those
+// run one workgroup (two waves per SIMD) per CU.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
 // Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and for

@@ -675,6 +675,85 @@ int flutas_psolve_solve(flutas_psolve *h, double *p) {
   return check_nonfinite(h);
 }
 
+// ---- graphs of library calls ------------------------------------------------
+// No counterpart in the reference.  Between flutas_graph_begin and
+// flutas_graph_end, the asynchronous library calls (FP64 solves through
+// flutas_psolve_solve_async, the field kernels, boundp, bounduvw without
+// outflow faces) are recorded from the library stream instead of run; the
+// graph then replays the whole sequence as one launch.  Calls that
+// synchronise or allocate (the synchronous solves, the refined mode,
+// chkdiv) fail the capture.  The null stream cannot be captured, so a
+// library stream left at the default is replaced by a private stream for
+// the capture.
+struct flutas_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static hipStream_t g_capture_stream = nullptr;  // private capture stream
+static hipStream_t g_saved_stream = nullptr;
+static bool g_capturing = false;
+
+int flutas_graph_begin(void) {
+  if (g_capturing) return fail(FLUTAS_EINVAL, "a graph capture is already open");
+  hipStream_t s = g_stream;
+  if (!s) {
+    if (!g_capture_stream)
+      FL_HIP(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
+    // the capture starts after the work already queued on the null stream
+    FL_HIP(hipStreamSynchronize(nullptr));
+    s = g_capture_stream;
+  }
+  FL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  g_saved_stream = g_stream;
+  g_stream = s;
+  g_capturing = true;
+  return FLUTAS_OK;
+}
+
+int flutas_graph_end(flutas_graph **out) {
+  if (!out) return fail(FLUTAS_EINVAL, "null pointer");
+  *out = nullptr;
+  if (!g_capturing) return fail(FLUTAS_EINVAL, "no graph capture is open");
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(g_stream, &g);
+  g_stream = g_saved_stream;
+  g_capturing = false;
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    // the call that broke the capture left its error as the thread's last
+    // HIP error; clear it, or the next launch check would report it
+    (void)hipGetLastError();
+    return fail(FLUTAS_EINVAL, std::string("graph capture failed (a call synchronised or "
+                                           "allocated?): ") + hipGetErrorString(e));
+  }
+  auto *fg = new flutas_graph;
+  fg->graph = g;
+  const hipError_t ei = hipGraphInstantiate(&fg->exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    delete fg;
+    return fail(FLUTAS_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  *out = fg;
+  return FLUTAS_OK;
+}
+
+int flutas_graph_launch(flutas_graph *g) {
+  if (!g) return fail(FLUTAS_EINVAL, "null graph");
+  if (g_capturing) return fail(FLUTAS_EINVAL, "a graph capture is open");
+  FL_HIP(hipGraphLaunch(g->exec, g_stream));
+  return FLUTAS_OK;
+}
+
+int flutas_graph_destroy(flutas_graph *g) {
+  if (!g) return FLUTAS_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return FLUTAS_OK;
+}
+
 int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
                           int reps, int max_stages, double *ms, int *nstages) {
   if (!h || !f || !p || !ms || !nstages || reps < 1) return fail(FLUTAS_EINVAL, "bad profile arguments");

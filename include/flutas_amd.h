@@ -135,6 +135,23 @@ int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
                           int reps, int max_stages, double *ms, int *nstages);
 const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 
+/* ---- graphs of library calls (no counterpart in the reference, whose
+ *      steps launch kernel by kernel) --------------------------------------
+ * Between flutas_graph_begin and flutas_graph_end the asynchronous library
+ * calls are recorded instead of run: FP64 solves through
+ * flutas_psolve_solve_async / the step's field kernels (fillps, fillps_rhs,
+ * updt_rhs_b, pres_*_src, correc, correc_update, copy/add_interior), boundp
+ * and bounduvw without outflow faces.  Calls that synchronise or allocate
+ * (synchronous solves, the refined mode, chkdiv, bounduvw with outflow)
+ * make flutas_graph_end fail.  flutas_graph_launch replays the recorded
+ * sequence on the library stream (asynchronous).  A non-finite right-hand
+ * side met in a replayed solve is not reported. */
+typedef struct flutas_graph flutas_graph;
+int flutas_graph_begin(void);
+int flutas_graph_end(flutas_graph **g);
+int flutas_graph_launch(flutas_graph *g);
+int flutas_graph_destroy(flutas_graph *g);
+
 /* fft (src/fft.f90:181-193): executes one r2r kind in place on `howmany`
  * device lines of length n (element stride `stride`, line distance
  * `dist`), FP64, unnormalised FFTW definitions.  kind uses FFTW's
@@ -355,6 +372,22 @@ int flutas_correc_update(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
                          double rho0, const double *phi, const double *pold,
                          double *pnew, double *u, double *v, double *w,
                          const double *rho, int const_coeff);
+
+/* flutas_correc_update with the two boundp calls around it folded in
+ * (main__single_phase.f90:727-747: boundp(phi) before correc, boundp(p)
+ * after the update; src/bound.f90:146-225 with nh_p = 1): the halo values
+ * of phi that correc reads are computed from phi's interior with set_bc's
+ * rules (phi's halo is neither read nor written), and every halo point of
+ * pnew is set as boundp sets it.  cbc, bc, dl, dzc_0, dzc_n as for
+ * flutas_boundp.  Bit for bit boundp(phi), flutas_correc_update,
+ * boundp(pnew), in one pass. */
+int flutas_correc_update_bc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                            double dyi, double dzi, const double *dzci, double dt,
+                            double rho0, const double *phi, const double *pold,
+                            double *pnew, double *u, double *v, double *w,
+                            const double *rho, int const_coeff, const char cbc[6],
+                            const double bc[6], const double dl[3], double dzc_0,
+                            double dzc_n);
 
 /* pold = p on the interior (main__two_phase_inc_isot.f90:656-673). */
 int flutas_copy_interior(int nx, int ny, int nz, const double *src, double *dst);

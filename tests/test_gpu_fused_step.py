@@ -84,6 +84,45 @@ def test_correc_update_equals_correc_then_update(api, oracle, nh_u, cc):
             np.testing.assert_array_equal(poldd.to_host(), pold)
 
 
+@pytest.mark.parametrize("cbc,nh_u,cc", [("PPPPNN", 1, True), ("NNDDPP", 2, False),
+                                          ("DDNNDN", 1, True), ("PPNNND", 2, True),
+                                          ("DNPPDD", 1, False), ("PPPPPP", 1, True)])
+def test_correc_update_bc_equals_boundp_correc_update_boundp(api, oracle, cbc, nh_u, cc):
+    """flutas_correc_update_bc against boundp(phi), correc_update,
+    boundp(pnew) (the step's order around the solve), bit for bit on every
+    point of u, v, w and pnew (halos, edges and corners included), with
+    nonzero boundary values, in place and out of place; phi stays as it
+    was, garbage halo included."""
+    ng = (37, 20, 11)
+    nx, ny, nz = ng
+    nh_d = 1
+    dzc, _, dzci, _, _, _ = z_metrics(oracle, nz, 1.3, 1.0)
+    dli = (nx / 1.1, ny / 0.7, float(nz))
+    dl = tuple(1.0 / d for d in dli)
+    bc = tuple(b if c != "P" else 0.0 for b, c in zip((0.2, -0.1, 0.3, 0.4, 0.7, -0.3), cbc))
+    u, v, w, phi, pold, rho = _fields(ng, 7, nh_u)
+    print(f"correc_update_bc {ng} {cbc} nh_u={nh_u} const_coeff={cc}", flush=True)
+    D = api.DeviceArray.from_host
+    dzci_d = D(dzci)
+    for in_place in (True, False):
+        # the three calls
+        ua, va, wa, phia, polda = D(u), D(v), D(w), D(phi), D(pold)
+        pnewa = polda if in_place else D(np.zeros_like(pold))
+        api.boundp(cbc, ng, bc, dl, dzc[0], dzc[nz], phia)
+        api.correc_update(nx, ny, nz, nh_d, nh_u, dli[0], dli[1], dli[2], dzci_d, 0.3, 2.5,
+                          phia, polda, pnewa, ua, va, wa, rho=D(rho), const_coeff=cc)
+        api.boundp(cbc, ng, bc, dl, dzc[0], dzc[nz], pnewa)
+        # one pass
+        ub, vb, wb, phib, poldb = D(u), D(v), D(w), D(phi), D(pold)
+        pnewb = poldb if in_place else D(np.zeros_like(pold))
+        api.correc_update_bc(nx, ny, nz, nh_d, nh_u, dli[0], dli[1], dli[2], dzci_d, 0.3, 2.5,
+                             phib, poldb, pnewb, ub, vb, wb, cbc, bc, dl, dzc[0], dzc[nz],
+                             rho=D(rho), const_coeff=cc)
+        for a, b in ((ub, ua), (vb, va), (wb, wa), (pnewb, pnewa)):
+            np.testing.assert_array_equal(a.to_host(), b.to_host())
+        np.testing.assert_array_equal(phib.to_host(), phi)
+
+
 def test_correc_update_rejects_aliased_increment(api):
     a = api.DeviceArray((6, 6, 6))
     u = api.DeviceArray((6, 6, 6))

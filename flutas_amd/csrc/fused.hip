@@ -84,8 +84,8 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
 
 bool fused_supported(int n) {
   switch (n / 2) {
-    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192:
-    case 256: case 384: case 512: case 1024:
+    case 16: case 24: case 32: case 40: case 48: case 64: case 80: case 96: case 128:
+    case 160: case 192: case 256: case 320: case 384: case 512: case 1024:
       return (n % 2) == 0;
     default:
       return false;
@@ -97,12 +97,16 @@ bool fused_supported(int n) {
     case 16: { constexpr int NC = 16; __VA_ARGS__; } break;   \
     case 24: { constexpr int NC = 24; __VA_ARGS__; } break;   \
     case 32: { constexpr int NC = 32; __VA_ARGS__; } break;   \
+    case 40: { constexpr int NC = 40; __VA_ARGS__; } break;   \
     case 48: { constexpr int NC = 48; __VA_ARGS__; } break;   \
     case 64: { constexpr int NC = 64; __VA_ARGS__; } break;   \
+    case 80: { constexpr int NC = 80; __VA_ARGS__; } break;   \
     case 96: { constexpr int NC = 96; __VA_ARGS__; } break;   \
     case 128: { constexpr int NC = 128; __VA_ARGS__; } break; \
+    case 160: { constexpr int NC = 160; __VA_ARGS__; } break; \
     case 192: { constexpr int NC = 192; __VA_ARGS__; } break; \
     case 256: { constexpr int NC = 256; __VA_ARGS__; } break; \
+    case 320: { constexpr int NC = 320; __VA_ARGS__; } break; \
     case 384: { constexpr int NC = 384; __VA_ARGS__; } break; \
     case 512: { constexpr int NC = 512; __VA_ARGS__; } break; \
     case 1024: { constexpr int NC = 1024; __VA_ARGS__; } break; \
@@ -207,12 +211,16 @@ int fused_group_planes(int ny, int elem_bytes) {
       case 16: return group_planes<T, 16>();
       case 24: return group_planes<T, 24>();
       case 32: return group_planes<T, 32>();
+      case 40: return group_planes<T, 40>();
       case 48: return group_planes<T, 48>();
       case 64: return group_planes<T, 64>();
+      case 80: return group_planes<T, 80>();
       case 96: return group_planes<T, 96>();
       case 128: return group_planes<T, 128>();
+      case 160: return group_planes<T, 160>();
       case 192: return group_planes<T, 192>();
       case 256: return group_planes<T, 256>();
+      case 320: return group_planes<T, 320>();
       case 384: return group_planes<T, 384>();
       case 512: return group_planes<T, 512>();
       case 1024: return group_planes<T, 1024>();

@@ -118,17 +118,45 @@ __device__ __forceinline__ void d16(cx<T> *v) {
   }
 }
 
+// 5-point DFT: with t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3
+// and w = exp(S 2 pi i / 5),
+//   X0 = x0 + t1 + t2,
+//   X1, X4 = x0 + c1 t1 + c2 t2 +- S i (s1 t3 + s2 t4),
+//   X2, X3 = x0 + c2 t1 + c1 t2 +- S i (s2 t3 - s1 t4),
+// c1 = cos(2 pi/5), c2 = cos(4 pi/5), s1 = sin(2 pi/5), s2 = sin(4 pi/5).
+template <typename T, int S>
+__device__ __forceinline__ void d5(cx<T> *v) {
+  const T c1 = T(0.309016994374947424102293417182819059L);
+  const T c2 = T(-0.809016994374947424102293417182819059L);
+  const T s1 = T(0.951056516295153572116439333379382143L);
+  const T s2 = T(0.587785252292473129168705954639072769L);
+  const cx<T> t1 = add(v[1], v[4]), t2 = add(v[2], v[3]);
+  const cx<T> t3 = sub(v[1], v[4]), t4 = sub(v[2], v[3]);
+  const cx<T> x0 = v[0];
+  const cx<T> a1 = {x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y};
+  const cx<T> a2 = {x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y};
+  const cx<T> b1 = msi<T, S>(cx<T>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+  const cx<T> b2 = msi<T, S>(cx<T>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+  v[0] = add(x0, add(t1, t2));
+  v[1] = add(a1, b1);
+  v[4] = sub(a1, b1);
+  v[2] = add(a2, b2);
+  v[3] = sub(a2, b2);
+}
+
 template <typename T, int S, int R>
 __device__ __forceinline__ void dft(cx<T> *v) {
   if constexpr (R == 2) d2<T, S>(v);
   else if constexpr (R == 3) d3<T, S>(v);
+  else if constexpr (R == 5) d5<T, S>(v);
   else if constexpr (R == 4) d4<T, S>(v);
   else if constexpr (R == 8) d8<T, S>(v);
   else if constexpr (R == 16) d16<T, S>(v);
   else static_assert(R == 2, "radix not built");
 }
 
-// Compile-time radix plans of the complex length N = n/2.  Pass s reads
+// Compile-time radix plans of the complex length N = n/2 (radix 16/8/4 first,
+// then 3 or 5).  Pass s reads
 // with stride N/r[s] and writes with stride ns(s) = r[0]*...*r[s-1].
 // Twiddles of pass s >= 1 live in their own table laid out [rr][j]
 // (rr < r[s], j < ns(s)) so that lanes with consecutive j read
@@ -156,6 +184,11 @@ FL_PLAN(256, 8, 8, 4, 3)
 FL_PLAN(384, 16, 8, 3, 3)
 FL_PLAN(512, 8, 8, 8, 3)
 FL_PLAN(1024, 16, 16, 4, 3)
+// radix-5 lengths (real lines of 80, 160, 320 and 640 points)
+FL_PLAN(40, 8, 5, 1, 2)
+FL_PLAN(80, 16, 5, 1, 2)
+FL_PLAN(160, 8, 4, 5, 3)
+FL_PLAN(320, 8, 8, 5, 3)
 #undef FL_PLAN
 
 // XOR swizzle of the buffer between passes 0 and 1 (power-of-two r[0]):

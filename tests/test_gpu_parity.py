@@ -102,6 +102,12 @@ SOLVE_CASES = [
     ((1024, 32, 6), (1.0, 1.0, 1.0), "DDPPNN", 0.0),
     ((32, 1024, 12), (1.0, 1.0, 1.0), "PPPPPP", 0.0),
     ((2048, 32, 4), (1.0, 1.0, 1.0), "NNPPNN", 0.0),
+    # radix-5 lengths on the fused pipeline (80, 160, 320, 640 points)
+    ((80, 160, 12), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((320, 80, 8), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((640, 32, 6), (1.0, 1.0, 1.0), "DDPPNN", 0.0),
+    ((32, 640, 8), (1.0, 1.0, 2.0), "PPPPPP", 0.0),
+    ((160, 320, 10), (1.0, 1.0, 1.0), "NNDDDD", 1.2),
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),

@@ -30,18 +30,19 @@ static inline dim3 grid3(int nx, int ny, int nz, int bx) {
 }
 
 // ---- fillps (src/fillps.f90:42-61, _CONSTANT_COEFFS_POISSON) -------------
+template <typename T>
 __global__ void __launch_bounds__(kRowThreads)
-    k_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
-             const double *__restrict__ dzfi, double dti, double rho0,
-             const double *__restrict__ u, const double *__restrict__ v,
-             const double *__restrict__ w, double *__restrict__ p) {
+    k_fillps(int nx, int ny, int nz, int nh_d, int nh_u, T dxi, T dyi,
+             const T *__restrict__ dzfi, T dti, T rho0,
+             const T *__restrict__ u, const T *__restrict__ v,
+             const T *__restrict__ w, T *__restrict__ p) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > nx) return;
-  const double dtidxi = dti * dxi, dtidyi = dti * dyi;
+  const T dtidxi = dti * dxi, dtidyi = dti * dyi;
   const long long c = ixh(i, j, k, nh_u, nx, ny);
-  const double val =
+  const T val =
       ((w[c] - w[ixh(i, j, k - 1, nh_u, nx, ny)]) * dti * dzfi[k + nh_d - 1] +
        (v[c] - v[ixh(i, j - 1, k, nh_u, nx, ny)]) * dtidyi) +
       (u[c] - u[c - 1]) * dtidxi;
@@ -49,13 +50,14 @@ __global__ void __launch_bounds__(kRowThreads)
 }
 
 // ---- pres_tw_src (src/source.f90:272-306) -------------------------------
+template <typename T>
 __global__ void __launch_bounds__(kRowThreads)
-    k_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi, int nh_d,
-                  int nh_u, const double *__restrict__ dzci, double rho0i,
-                  double f_t12, double f1, double f2,
-                  const double *__restrict__ p, const double *__restrict__ pold,
-                  const double *__restrict__ rho, double *__restrict__ u,
-                  double *__restrict__ v, double *__restrict__ w, int cc) {
+    k_pres_tw_src(int nx, int ny, int nz, T dxi, T dyi, int nh_d,
+                  int nh_u, const T *__restrict__ dzci, T rho0i,
+                  T f_t12, T f1, T f2,
+                  const T *__restrict__ p, const T *__restrict__ pold,
+                  const T *__restrict__ rho, T *__restrict__ u,
+                  T *__restrict__ v, T *__restrict__ w, int cc) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
@@ -63,14 +65,14 @@ __global__ void __launch_bounds__(kRowThreads)
   const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
   const long long ip = c1 + 1, jp = c1 + (nx + 2),
                   kp = c1 + (long long)(nx + 2) * (ny + 2);
-  const double rc = rho[c1];
-  const double rhoxi = 1.0 / (0.5 * (rho[ip] + rc));
-  const double rhoyi = 1.0 / (0.5 * (rho[jp] + rc));
-  const double rhozi = 1.0 / (0.5 * (rho[kp] + rc));
-  const double dzk = dzci[k + nh_d - 1];
-  const double pc = p[c1];
+  const T rc = rho[c1];
+  const T rhoxi = T(1.0) / (T(0.5) * (rho[ip] + rc));
+  const T rhoyi = T(1.0) / (T(0.5) * (rho[jp] + rc));
+  const T rhozi = T(1.0) / (T(0.5) * (rho[kp] + rc));
+  const T dzk = dzci[k + nh_d - 1];
+  const T pc = p[c1];
   if (cc) {
-    const double hc = f1 * pc - f2 * pold[c1];
+    const T hc = f1 * pc - f2 * pold[c1];
     u[cu] = u[cu] + f_t12 * ((-((p[ip] - pc) * dxi)) * rho0i -
                              (rhoxi - rho0i) * ((f1 * p[ip] - f2 * pold[ip]) - hc) * dxi);
     v[cu] = v[cu] + f_t12 * ((-((p[jp] - pc) * dyi)) * rho0i -
@@ -85,18 +87,19 @@ __global__ void __launch_bounds__(kRowThreads)
 }
 
 // ---- pres_sp_src (src/source.f90:326-343) -------------------------------
+template <typename T>
 __global__ void __launch_bounds__(kRowThreads)
-    k_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi, double dyi,
-                  int nh_d, int nh_u, const double *__restrict__ dzci,
-                  double rho0i, const double *__restrict__ pold,
-                  double *__restrict__ u, double *__restrict__ v,
-                  double *__restrict__ w) {
+    k_pres_sp_src(int nx, int ny, int nz, T f_t12, T dxi, T dyi,
+                  int nh_d, int nh_u, const T *__restrict__ dzci,
+                  T rho0i, const T *__restrict__ pold,
+                  T *__restrict__ u, T *__restrict__ v,
+                  T *__restrict__ w) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
   if (i > nx) return;
   const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
-  const double pc = pold[c1];
+  const T pc = pold[c1];
   u[cu] = u[cu] + f_t12 * (-((pold[c1 + 1] - pc) * dxi)) * rho0i;
   v[cu] = v[cu] + f_t12 * (-((pold[c1 + (nx + 2)] - pc) * dyi)) * rho0i;
   w[cu] = w[cu] + f_t12 * (-((pold[c1 + (long long)(nx + 2) * (ny + 2)] - pc) *
@@ -104,12 +107,13 @@ __global__ void __launch_bounds__(kRowThreads)
 }
 
 // ---- correc (src/correc.f90:120-144) ------------------------------------
+template <typename T>
 __global__ void __launch_bounds__(kRowThreads)
-    k_correc(int nx, int ny, int nz, int nh_d, int nh_u, double factori,
-             double factorj, const double *__restrict__ dzci, double dt,
-             double rho0i, const double *__restrict__ p, double *__restrict__ u,
-             double *__restrict__ v, double *__restrict__ w,
-             const double *__restrict__ rho, int cc) {
+    k_correc(int nx, int ny, int nz, int nh_d, int nh_u, T factori,
+             T factorj, const T *__restrict__ dzci, T dt,
+             T rho0i, const T *__restrict__ p, T *__restrict__ u,
+             T *__restrict__ v, T *__restrict__ w,
+             const T *__restrict__ rho, int cc) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y + 1, k = blockIdx.z + 1;
@@ -117,16 +121,16 @@ __global__ void __launch_bounds__(kRowThreads)
   const long long c1 = ixh(i, j, k, 1, nx, ny), cu = ixh(i, j, k, nh_u, nx, ny);
   const long long ip = c1 + 1, jp = c1 + (nx + 2),
                   kp = c1 + (long long)(nx + 2) * (ny + 2);
-  const double pc = p[c1];
-  const double fz = dt * dzci[k + nh_d - 1];
+  const T pc = p[c1];
+  const T fz = dt * dzci[k + nh_d - 1];
   if (cc) {
     u[cu] = u[cu] - factori * (p[ip] - pc) * rho0i;
     v[cu] = v[cu] - factorj * (p[jp] - pc) * rho0i;
     w[cu] = w[cu] - fz * (p[kp] - pc) * rho0i;
   } else {
-    const double rc = rho[c1];
-    const double rhox = 0.5 * (rho[ip] + rc), rhoy = 0.5 * (rho[jp] + rc),
-                 rhoz = 0.5 * (rho[kp] + rc);
+    const T rc = rho[c1];
+    const T rhox = T(0.5) * (rho[ip] + rc), rhoy = T(0.5) * (rho[jp] + rc),
+                 rhoz = T(0.5) * (rho[kp] + rc);
     u[cu] = u[cu] - factori * (p[ip] - pc) / rhox;
     v[cu] = v[cu] - factorj * (p[jp] - pc) / rhoy;
     w[cu] = w[cu] - fz * (p[kp] - pc) / rhoz;
@@ -295,10 +299,11 @@ __global__ void __launch_bounds__(kRowThreads)
 // ---- chkdiv (src/chkdiv.f90:38-60): per-block partial sum and max -------
 constexpr int kRedThreads = 256;
 
+template <typename T>
 __global__ void __launch_bounds__(kRedThreads)
-    k_chkdiv(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
-             const double *__restrict__ dzfi, const double *__restrict__ u,
-             const double *__restrict__ v, const double *__restrict__ w,
+    k_chkdiv(int nx, int ny, int nz, int nh_d, int nh_u, T dxi, T dyi,
+             const T *__restrict__ dzfi, const T *__restrict__ u,
+             const T *__restrict__ v, const T *__restrict__ w,
              double *__restrict__ part) {
 #pragma clang fp contract(off)
   __shared__ double ssum[kRedThreads], smax[kRedThreads];
@@ -310,12 +315,13 @@ __global__ void __launch_bounds__(kRedThreads)
     const long long r = t / nx;
     const int j = (int)(r % ny) + 1, k = (int)(r / ny) + 1;
     const long long c = ixh(i, j, k, nh_u, nx, ny);
-    const double div =
+    // the divergence in the fields' precision, the sums in FP64
+    const T div =
         ((w[c] - w[ixh(i, j, k - 1, nh_u, nx, ny)]) * dzfi[k + nh_d - 1] +
          (v[c] - v[ixh(i, j - 1, k, nh_u, nx, ny)]) * dyi) +
         (u[c] - u[c - 1]) * dxi;
-    s += div;
-    const double ad = fabs(div);
+    s += (double)div;
+    const double ad = fabs((double)div);
     m = (ad > m || ad != ad) ? ad : m;
   }
   ssum[threadIdx.x] = s;
@@ -375,11 +381,12 @@ static int chkdiv_blocks(long long total) {
 // Enqueues chkdiv on stream s with the partials and the two results in
 // `work` (2 * chkdiv_blocks + 2 doubles, results last) and copies the
 // results to the host values; the caller synchronises.
-static int chkdiv_enqueue(int nx, int ny, int nz, int nh_d, int nh_u, double dxi, double dyi,
-                          const double *dzfi, const double *u, const double *v,
-                          const double *w, double *work, double *res_host, hipStream_t s) {
+template <typename T>
+static int chkdiv_enqueue(int nx, int ny, int nz, int nh_d, int nh_u, T dxi, T dyi,
+                          const T *dzfi, const T *u, const T *v,
+                          const T *w, double *work, double *res_host, hipStream_t s) {
   const int nb = chkdiv_blocks((long long)nx * ny * nz);
-  hipLaunchKernelGGL(k_chkdiv, dim3(nb), dim3(kRedThreads), 0, s, nx, ny, nz, nh_d, nh_u, dxi,
+  hipLaunchKernelGGL(k_chkdiv<T>, dim3(nb), dim3(kRedThreads), 0, s, nx, ny, nz, nh_d, nh_u, dxi,
                      dyi, dzfi, u, v, w, work);
   FL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(kRedThreads), 0, s, work, nb, work + 2 * nb);
@@ -426,8 +433,9 @@ static int chkdiv_scratch_ptr(double **ptr) {
 // One face: dir 0/1/2, side 0/1; type 'P' periodic copy, 'D' centred
 // Dirichlet p0 = 2v - p1, 'N' p0 = p1 -+ dr v.  Loops over the full
 // tangential extent including the halo, as set_bc does.
+template <typename T>
 __global__ void k_set_bc(int nx, int ny, int nz, char ctype, int side, int dir,
-                         double value, double dr, double *__restrict__ p) {
+                         T value, T dr, T *__restrict__ p) {
   const int n[3] = {nx, ny, nz};
   const long long st[3] = {1, nx + 2, (long long)(nx + 2) * (ny + 2)};
   const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
@@ -440,9 +448,9 @@ __global__ void k_set_bc(int nx, int ny, int nz, char ctype, int side, int dir,
     else p[base + (nd + 1) * s] = p[base + s];
     return;
   }
-  double factor = value, sgn = 0.0;
-  if (ctype == 'D') { factor = 2.0 * factor; sgn = -1.0; }
-  if (ctype == 'N') { factor = (side == 0) ? -dr * factor : dr * factor; sgn = 1.0; }
+  T factor = value, sgn = T(0.0);
+  if (ctype == 'D') { factor = T(2.0) * factor; sgn = -T(1.0); }
+  if (ctype == 'N') { factor = (side == 0) ? -dr * factor : dr * factor; sgn = T(1.0); }
   if (side == 0) p[base] = factor + sgn * p[base + s];
   else p[base + (nd + 1) * s] = factor + sgn * p[base + nd * s];
 }
@@ -450,15 +458,17 @@ __global__ void k_set_bc(int nx, int ny, int nz, char ctype, int side, int dir,
 // ---- bounduvw: set_bc on u, v, w with halo nh, and outflow ---------------
 // (src/bound.f90:17-144, :227-646, :649-773)
 constexpr int kMaxHalo = 8;
+template <typename T>
 struct FaceField {
-  double *p;
-  double dr[kMaxHalo];  // spacing of layer q (Neumann factor)
-  double value;
+  T *p;
+  T dr[kMaxHalo];  // spacing of layer q (Neumann factor)
+  T value;
   char ctype;           // 'P', 'D', 'N'; 0: field not on this face
   int centered;         // cell-centred component (else face-normal)
 };
+template <typename T>
 struct FaceArgs {
-  FaceField f[3];
+  FaceField<T> f[3];
   int n[3];
   int nh, dir, side;
 };
@@ -469,7 +479,8 @@ struct FaceArgs {
 // overwrites an earlier one exactly where the reference's does (the upper
 // side of the face-normal kinds writes positions nd + q and nd + 1 + q).
 // Reads are interior positions of the line only, so lanes are independent.
-__global__ void k_set_bc_face(FaceArgs a) {
+template <typename T>
+__global__ void k_set_bc_face(FaceArgs<T> a) {
 #pragma clang fp contract(off)
   const int d = a.dir, t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
   const int nh = a.nh, nd = a.n[d];
@@ -482,15 +493,15 @@ __global__ void k_set_bc_face(FaceArgs a) {
   for (int c = 0; c < 3; ++c) {
     const char t = a.f[c].ctype;
     if (!t) continue;
-    double *p = a.f[c].p;
-    auto L = [&](int pos) -> double & { return p[base + (long long)(pos - 1 + nh) * s]; };
+    T *p = a.f[c].p;
+    auto L = [&](int pos) -> T & { return p[base + (long long)(pos - 1 + nh) * s]; };
     const bool cen = a.f[c].centered != 0;
-    double sgn = 0.0;
-    if (t == 'D' && cen) sgn = -1.0;
-    if (t == 'N') sgn = +1.0;
+    T sgn = T(0.0);
+    if (t == 'D' && cen) sgn = -T(1.0);
+    if (t == 'N') sgn = +T(1.0);
     for (int q = 0; q < nh; ++q) {
-      double f = a.f[c].value;
-      if (t == 'D' && cen) f = +2.0 * f;
+      T f = a.f[c].value;
+      if (t == 'D' && cen) f = +T(2.0) * f;
       if (t == 'N') f = (a.side == 0) ? -a.f[c].dr[q] * f : a.f[c].dr[q] * f;
       if (t == 'P') {
         L(0 - q) = L(nd - q);
@@ -507,10 +518,10 @@ __global__ void k_set_bc_face(FaceArgs a) {
         }
       } else {
         if (a.side == 0) {
-          L(0 - q) = 1.0 * f + L(1 + q);
+          L(0 - q) = T(1.0) * f + L(1 + q);
         } else {
-          L(nd + q) = 1.0 * f + L(nd - 1 - q);
-          L(nd + 1 + q) = 2.0 * f + L(nd - 1 - q);
+          L(nd + q) = T(1.0) * f + L(nd - 1 - q);
+          L(nd + 1 + q) = T(2.0) * f + L(nd - 1 - q);
         }
       }
     }
@@ -522,28 +533,30 @@ __global__ void k_set_bc_face(FaceArgs a) {
 // the lower side; one lane per tangential interior position.  dzf: device
 // (1-nh_d : nz+nh_d) for the x and y faces; dzq: dzf at the z-face layers
 // (upper: dzf(nz+q); lower: dzf(-q)).
+template <typename T>
 struct OutflowArgs {
-  double *u, *v, *w;
-  const double *dzf;
-  double dzq[kMaxHalo + 1];
-  double dx, dy, dxi, dyi;
+  T *u, *v, *w;
+  const T *dzf;
+  T dzq[kMaxHalo + 1];
+  T dx, dy, dxi, dyi;
   int n[3];
   int nh, nh_d, dir, side;
 };
-__global__ void k_outflow(OutflowArgs a) {
+template <typename T>
+__global__ void k_outflow(OutflowArgs<T> a) {
 #pragma clang fp contract(off)
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2], nh = a.nh;
   const long long sx = nx + 2LL * nh, sy = sx * (ny + 2LL * nh);
-  auto F = [&](double *f, int i, int j, int k) -> double & {
+  auto F = [&](T *f, int i, int j, int k) -> T & {
     return f[(i - 1 + nh) + sx * (j - 1 + nh) + sy * (k - 1 + nh)];
   };
   const int t1 = blockIdx.x * blockDim.x + threadIdx.x + 1, t2 = blockIdx.y + 1;
   const int qmin = nh - 1;
-  double *u = a.u, *v = a.v, *w = a.w;
+  T *u = a.u, *v = a.v, *w = a.w;
   if (a.dir == 0) {
     const int j = t1, k = t2;
     if (j > ny || k > nz) return;
-    const double dzfi = 1.0 / a.dzf[k + a.nh_d - 1];
+    const T dzfi = T(1.0) / a.dzf[k + a.nh_d - 1];
     if (a.side == 1) {
       const int i = nx;
       for (int q = 0; q <= qmin + 1; ++q)
@@ -560,7 +573,7 @@ __global__ void k_outflow(OutflowArgs a) {
   } else if (a.dir == 1) {
     const int i = t1, k = t2;
     if (i > nx || k > nz) return;
-    const double dzfi = 1.0 / a.dzf[k + a.nh_d - 1];
+    const T dzfi = T(1.0) / a.dzf[k + a.nh_d - 1];
     if (a.side == 1) {
       const int j = ny;
       for (int q = 0; q <= qmin + 1; ++q)
@@ -594,9 +607,10 @@ __global__ void k_outflow(OutflowArgs a) {
 }
 
 // ---- updt_rhs_b (src/bound.f90:829-944) ------------------------------------
+template <typename T>
 __global__ void k_add_face(int nx, int ny, int nz, int dir, int q,
-                           const double *__restrict__ rhs,
-                           double *__restrict__ p) {
+                           const T *__restrict__ rhs,
+                           T *__restrict__ p) {
   const int n[3] = {nx, ny, nz};
   const long long st[3] = {1, nx + 2, (long long)(nx + 2) * (ny + 2)};
   const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
@@ -893,95 +907,381 @@ static int check_dims(int nx, int ny, int nz) {
   return FLUTAS_OK;
 }
 
-extern "C" {
-
-int flutas_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
-                  double dyi, double dzi, const double *dzfi, double dti,
-                  double rho0, const double *u, const double *v,
-                  const double *w, double *p) {
+// ---- host sides of the field entry points, for FP64 and FP32 fields ----
+template <typename T>
+static int chkdiv_impl(int nx, int ny, int nz, T dxi, T dyi, T dzi, int nh_d, int nh_u,
+                       const T *dzfi, const T *u, const T *v, const T *w, double res[2]) {
+  (void)dzi;
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!dzfi || !u || !v || !w || !res) return fail(FLUTAS_EINVAL, "null pointer");
+  hipStream_t s = current_stream();
+  const int nb = chkdiv_blocks((long long)nx * ny * nz);
+  double *work = nullptr;
+  FL_HIP(hipMallocAsync((void **)&work, sizeof(double) * (2 * nb + 2), s));
+  res[0] = res[1] = 0.0;
+  const int rc = chkdiv_enqueue<T>(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, work, res, s);
+  FL_HIP(hipFreeAsync(work, s));
+  FL_HIP(hipStreamSynchronize(s));
+  return rc;
+}
+template <typename T>
+static int fillps_impl(int nx, int ny, int nz, int nh_d, int nh_u, T dxi,
+                  T dyi, T dzi, const T *dzfi, T dti,
+                  T rho0, const T *u, const T *v,
+                  const T *w, T *p) {
   (void)dzi;
   FL_TRY(check_dims(nx, ny, nz));
   if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
   if (!dzfi || !u || !v || !w || !p) return fail(FLUTAS_EINVAL, "null pointer");
   const int bx = xblock(nx);
-  hipLaunchKernelGGL(k_fillps, grid3(nx, ny, nz, bx), dim3(bx), 0,
+  hipLaunchKernelGGL(k_fillps<T>, grid3(nx, ny, nz, bx), dim3(bx), 0,
                      current_stream(), nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi,
                      dti, rho0, u, v, w, p);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
-
-int flutas_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi,
-                       double dzi, int nh_d, int nh_u, const double *dzci,
-                       double rho0i, double f_t12, double f_t12_o,
-                       const double *p, const double *pold, const double *rho,
-                       double *u, double *v, double *w, int const_coeff) {
+template <typename T>
+static int pres_tw_src_impl(int nx, int ny, int nz, T dxi, T dyi,
+                       T dzi, int nh_d, int nh_u, const T *dzci,
+                       T rho0i, T f_t12, T f_t12_o,
+                       const T *p, const T *pold, const T *rho,
+                       T *u, T *v, T *w, int const_coeff) {
   (void)dzi;
   FL_TRY(check_dims(nx, ny, nz));
   if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
   if (!dzci || !p || !pold || !rho || !u || !v || !w) return fail(FLUTAS_EINVAL, "null pointer");
-  const double f1 = 1.0 + (f_t12 / f_t12_o), f2 = (f_t12 / f_t12_o);
+  const T f1 = T(1.0) + (f_t12 / f_t12_o), f2 = (f_t12 / f_t12_o);
   const int bx = xblock(nx);
-  hipLaunchKernelGGL(k_pres_tw_src, grid3(nx, ny, nz, bx), dim3(bx), 0,
+  hipLaunchKernelGGL(k_pres_tw_src<T>, grid3(nx, ny, nz, bx), dim3(bx), 0,
                      current_stream(), nx, ny, nz, dxi, dyi, nh_d, nh_u, dzci,
                      rho0i, f_t12, f1, f2, p, pold, rho, u, v, w,
                      const_coeff ? 1 : 0);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
-
-int flutas_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi,
-                       double dyi, double dzi, int nh_d, int nh_u,
-                       const double *dzci, double rho0i, const double *pold,
-                       double *u, double *v, double *w) {
+template <typename T>
+static int pres_sp_src_impl(int nx, int ny, int nz, T f_t12, T dxi,
+                       T dyi, T dzi, int nh_d, int nh_u,
+                       const T *dzci, T rho0i, const T *pold,
+                       T *u, T *v, T *w) {
   (void)dzi;
   FL_TRY(check_dims(nx, ny, nz));
   if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
   if (!dzci || !pold || !u || !v || !w) return fail(FLUTAS_EINVAL, "null pointer");
   const int bx = xblock(nx);
-  hipLaunchKernelGGL(k_pres_sp_src, grid3(nx, ny, nz, bx), dim3(bx), 0,
+  hipLaunchKernelGGL(k_pres_sp_src<T>, grid3(nx, ny, nz, bx), dim3(bx), 0,
                      current_stream(), nx, ny, nz, f_t12, dxi, dyi, nh_d, nh_u,
                      dzci, rho0i, pold, u, v, w);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
-
-int flutas_correc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
-                  double dyi, double dzi, const double *dzci, double dt,
-                  double rho0, const double *p, double *u, double *v,
-                  double *w, const double *rho, int const_coeff) {
+template <typename T>
+static int correc_impl(int nx, int ny, int nz, int nh_d, int nh_u, T dxi,
+                  T dyi, T dzi, const T *dzci, T dt,
+                  T rho0, const T *p, T *u, T *v,
+                  T *w, const T *rho, int const_coeff) {
   (void)dzi;
   FL_TRY(check_dims(nx, ny, nz));
   if (nh_u < 1 || nh_d < 1) return fail(FLUTAS_EINVAL, "halo widths must be >= 1");
   if (!dzci || !p || !u || !v || !w || (!const_coeff && !rho))
     return fail(FLUTAS_EINVAL, "null pointer");
-  const double rho0i = 1.0 / rho0, factori = dt * dxi, factorj = dt * dyi;
+  const T rho0i = T(1.0) / rho0, factori = dt * dxi, factorj = dt * dyi;
   const int bx = xblock(nx);
-  hipLaunchKernelGGL(k_correc, grid3(nx, ny, nz, bx), dim3(bx), 0,
+  hipLaunchKernelGGL(k_correc<T>, grid3(nx, ny, nz, bx), dim3(bx), 0,
                      current_stream(), nx, ny, nz, nh_d, nh_u, factori, factorj,
                      dzci, dt, rho0i, p, u, v, w, rho, const_coeff ? 1 : 0);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
+template <typename T>
+static int boundp_impl(const char cbc[6], const int n[3], const T bc[6],
+                  const T dl[3], T dzc_0, T dzc_n, T *p) {
+  if (!cbc || !n || !bc || !dl || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  for (int q = 0; q < 6; ++q)
+    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N')
+      return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+  hipStream_t s = current_stream();
+  const int nx = n[0], ny = n[1], nz = n[2];
+  auto face = [&](char t, int side, int dir, T value, T dr) -> int {
+    const int ext[3] = {nx + 2, ny + 2, nz + 2};
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    dim3 grid((ext[t1] + 255) / 256, ext[t2]);
+    hipLaunchKernelGGL(k_set_bc<T>, grid, dim3(256), 0, s, nx, ny, nz, t, side,
+                       dir, value, dr, p);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  // periodic y and z halos first (the self-exchange of updthalo at one
+  // rank, src/bound.f90:182-183), then set_bc for x, y, z
+  if (cbc[2] == 'P') { FL_TRY(face('P', 0, 1, 0, 0)); FL_TRY(face('P', 1, 1, 0, 0)); }
+  if (cbc[4] == 'P') { FL_TRY(face('P', 0, 2, 0, 0)); FL_TRY(face('P', 1, 2, 0, 0)); }
+  FL_TRY(face(cbc[0], 0, 0, bc[0], dl[0]));
+  FL_TRY(face(cbc[1], 1, 0, bc[1], dl[0]));
+  if (cbc[2] != 'P') { FL_TRY(face(cbc[2], 0, 1, bc[2], dl[1])); FL_TRY(face(cbc[3], 1, 1, bc[3], dl[1])); }
+  if (cbc[4] != 'P') { FL_TRY(face(cbc[4], 0, 2, bc[4], dzc_0)); FL_TRY(face(cbc[5], 1, 2, bc[5], dzc_n)); }
+  return FLUTAS_OK;
+}
+template <typename T>
+static int bounduvw_impl(const char cbc[18], const int n[3], const T bc[18],
+                    int nh_d, int nh_u, const int isoutflow[6],
+                    const T dl[3], const T *dzc, const T *dzf,
+                    T *u, T *v, T *w) {
+  if (!cbc || !n || !bc || !isoutflow || !dl || !dzc || !dzf || !u || !v || !w)
+    return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
+  if (nh_u < 1 || nh_u > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_u must be in 1..8");
+  if (nh_d < nh_u) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_u (dzc, dzf at the halo layers)");
+  for (int d = 0; d < 3; ++d)
+    if (n[d] + 2 * nh_u > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
+  auto CB = [&](int s, int d, int c) { return cbc[s + 2 * d + 6 * c]; };
+  auto BV = [&](int s, int d, int c) { return bc[s + 2 * d + 6 * c]; };
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < 3; ++d) {
+      for (int s = 0; s < 2; ++s)
+        if (CB(s, d, c) != 'P' && CB(s, d, c) != 'D' && CB(s, d, c) != 'N')
+          return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+      if ((CB(0, d, c) == 'P') != (CB(1, d, c) == 'P'))
+        return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
+    }
+  hipStream_t s = current_stream();
+  T *fld[3] = {u, v, w};
+  const int nz = n[2];
+  auto Z = [&](const T *a, int k) { return a[k + nh_d - 1]; };
+  auto launch = [&](FaceArgs<T> &a) -> int {
+    const int t1 = (a.dir == 0) ? 1 : 0, t2 = (a.dir == 2) ? 1 : 2;
+    const int e1 = n[t1] + 2 * nh_u, e2 = n[t2] + 2 * nh_u;
+    hipLaunchKernelGGL(k_set_bc_face<T>, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  auto face = [&](int dir, int side) {
+    FaceArgs<T> a{};
+    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+    a.nh = nh_u;
+    a.dir = dir;
+    a.side = side;
+    return a;
+  };
+  // updthalo in y and z (periodic directions)
+  for (int d = 1; d <= 2; ++d) {
+    FaceArgs<T> a = face(d, 0);
+    for (int c = 0; c < 3; ++c)
+      if (CB(0, d, c) == 'P') {
+        a.f[c].p = fld[c];
+        a.f[c].ctype = 'P';
+        a.f[c].centered = 1;
+      }
+    if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
+  }
+  for (int d = 0; d < 3; ++d)
+    for (int side = 0; side < 2; ++side) {
+      FaceArgs<T> a = face(d, side);
+      for (int c = 0; c < 3; ++c) {
+        if (d > 0 && CB(side, d, c) == 'P') continue;  // done by updthalo
+        // a periodic set_bc copies both sides; its second call (side 1)
+        // would repeat the same copies
+        if (d == 0 && side == 1 && CB(side, d, c) == 'P') continue;
+        FaceField<T> &F = a.f[c];
+        F.p = fld[c];
+        F.ctype = CB(side, d, c);
+        F.centered = (c != d);
+        F.value = BV(side, d, c);
+        for (int q = 0; q < nh_u; ++q) {
+          if (d < 2) F.dr[q] = dl[d];
+          else if (c < 2) F.dr[q] = (side == 0) ? Z(dzc, -q) : Z(dzc, nz + q);
+          else F.dr[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
+        }
+      }
+      if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
+    }
+  // outflow faces
+  bool need_dzf = false;
+  for (int d = 0; d < 2; ++d) need_dzf = need_dzf || isoutflow[2 * d] || isoutflow[2 * d + 1];
+  T *dzf_d = nullptr;
+  const size_t nzh = (size_t)nz + 2 * nh_d;
+  if (need_dzf) {
+    FL_HIP(hipMalloc((void **)&dzf_d, sizeof(T) * nzh));
+    hipError_t e = hipMemcpy(dzf_d, dzf, sizeof(T) * nzh, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(dzf_d);
+      return fail(FLUTAS_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+  }
+  int rc = FLUTAS_OK;
+  for (int d = 0; d < 3 && rc == FLUTAS_OK; ++d)
+    for (int side = 0; side < 2 && rc == FLUTAS_OK; ++side) {
+      if (!isoutflow[side + 2 * d]) continue;
+      OutflowArgs<T> a{};
+      a.u = u; a.v = v; a.w = w;
+      a.dzf = dzf_d;
+      // layers written: 0..nh_u on the upper side, 0..nh_u-1 on the lower
+      for (int q = 0; q <= (side == 1 ? nh_u : nh_u - 1); ++q)
+        a.dzq[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
+      a.dx = dl[0]; a.dy = dl[1];
+      a.dxi = T(1.0) / dl[0]; a.dyi = T(1.0) / dl[1];
+      for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+      a.nh = nh_u; a.nh_d = nh_d; a.dir = d; a.side = side;
+      const int t1 = (d == 0) ? n[1] : n[0], t2 = (d == 2) ? n[1] : n[2];
+      hipLaunchKernelGGL(k_outflow<T>, dim3((t1 + 255) / 256, t2), dim3(256), 0, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = fail(FLUTAS_EDEVICE, std::string("k_outflow: ") + hipGetErrorString(e));
+    }
+  if (dzf_d) {
+    hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(dzf_d);
+    if (rc == FLUTAS_OK && e != hipSuccess) rc = fail(FLUTAS_EDEVICE, hipGetErrorString(e));
+  }
+  return rc;
+}
+template <typename T>
+static int boundp_halo_impl(const char cbc[6], const int n[3], const T bc[6],
+                       int nh_d, int nh_p, const T dl[3],
+                       const T *dzc, const T *dzf, T *p) {
+  (void)dzf;
+  if (!cbc || !n || !bc || !dl || !dzc || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(check_dims(n[0], n[1], n[2]));
+  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
+  if (nh_p < 1 || nh_p > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_p must be in 1..8");
+  if (nh_d < nh_p) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_p (dzc at the halo layers)");
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] + 2 * nh_p > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
+    for (int s = 0; s < 2; ++s)
+      if (cbc[2 * d + s] != 'P' && cbc[2 * d + s] != 'D' && cbc[2 * d + s] != 'N')
+        return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
+    if ((cbc[2 * d] == 'P') != (cbc[2 * d + 1] == 'P'))
+      return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
+  }
+  hipStream_t s = current_stream();
+  const int nz = n[2];
+  auto run = [&](int dir, int side, char t, T value, T dr0, bool zlo, bool zhi) -> int {
+    FaceArgs<T> a{};
+    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
+    a.nh = nh_p;
+    a.dir = dir;
+    a.side = side;
+    FaceField<T> &F = a.f[0];
+    F.p = p;
+    F.ctype = t;
+    F.centered = 1;
+    F.value = value;
+    for (int q = 0; q < nh_p; ++q)
+      F.dr[q] = zlo ? dzc[-q + nh_d - 1] : (zhi ? dzc[nz + q + nh_d - 1] : dr0);
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    const int e1 = n[t1] + 2 * nh_p, e2 = n[t2] + 2 * nh_p;
+    hipLaunchKernelGGL(k_set_bc_face<T>, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
+    FL_HIP(hipGetLastError());
+    return FLUTAS_OK;
+  };
+  // updthalo in y and z (periodic), then x, y, z faces as set_bc
+  if (cbc[2] == 'P') FL_TRY(run(1, 0, 'P', T(0.0), T(0.0), false, false));
+  if (cbc[4] == 'P') FL_TRY(run(2, 0, 'P', T(0.0), T(0.0), false, false));
+  FL_TRY(run(0, 0, cbc[0], bc[0], dl[0], false, false));
+  if (cbc[1] != 'P') FL_TRY(run(0, 1, cbc[1], bc[1], dl[0], false, false));  // P: both sides done
+  if (cbc[2] != 'P') FL_TRY(run(1, 0, cbc[2], bc[2], dl[1], false, false));
+  if (cbc[3] != 'P') FL_TRY(run(1, 1, cbc[3], bc[3], dl[1], false, false));
+  if (cbc[4] != 'P') FL_TRY(run(2, 0, cbc[4], bc[4], T(0.0), true, false));
+  if (cbc[5] != 'P') FL_TRY(run(2, 1, cbc[5], bc[5], T(0.0), false, true));
+  return FLUTAS_OK;
+}
+template <typename T>
+static int updt_rhs_b_impl(int nx, int ny, int nz, const char c_or_f[3],
+                      const char cbc[6], const T *rhsbx,
+                      const T *rhsby, const T *rhsbz, T *p) {
+  FL_TRY(check_dims(nx, ny, nz));
+  if (!c_or_f || !cbc || !rhsbx || !rhsby || !rhsbz || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  const int q1 = (c_or_f[0] == 'f' && cbc[1] == 'D') ? 1 : 0;
+  const int q2 = (c_or_f[1] == 'f' && cbc[3] == 'D') ? 1 : 0;
+  const int q3 = (c_or_f[2] == 'f' && cbc[5] == 'D') ? 1 : 0;
+  hipStream_t s = current_stream();
+  hipLaunchKernelGGL(k_add_face<T>, dim3((ny + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 0, q1, rhsbx, p);
+  hipLaunchKernelGGL(k_add_face<T>, dim3((nx + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 1, q2, rhsby, p);
+  hipLaunchKernelGGL(k_add_face<T>, dim3((nx + 255) / 256, ny), dim3(256), 0, s, nx, ny, nz, 2, q3, rhsbz, p);
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+extern "C" {
+
+int flutas_fillps(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzfi, double dti,
+                  double rho0, const double *u, const double *v,
+                  const double *w, double *p) {
+  return fillps_impl<double>(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w, p);
+}
+int flutas_fillps_f32(int nx, int ny, int nz, int nh_d, int nh_u, float dxi,
+                  float dyi, float dzi, const float *dzfi, float dti,
+                  float rho0, const float *u, const float *v,
+                  const float *w, float *p) {
+  return fillps_impl<float>(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w, p);
+}
+
+
+int flutas_pres_tw_src(int nx, int ny, int nz, double dxi, double dyi,
+                       double dzi, int nh_d, int nh_u, const double *dzci,
+                       double rho0i, double f_t12, double f_t12_o,
+                       const double *p, const double *pold, const double *rho,
+                       double *u, double *v, double *w, int const_coeff) {
+  return pres_tw_src_impl<double>(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, f_t12, f_t12_o, p, pold, rho, u, v, w, const_coeff);
+}
+int flutas_pres_tw_src_f32(int nx, int ny, int nz, float dxi, float dyi,
+                       float dzi, int nh_d, int nh_u, const float *dzci,
+                       float rho0i, float f_t12, float f_t12_o,
+                       const float *p, const float *pold, const float *rho,
+                       float *u, float *v, float *w, int const_coeff) {
+  return pres_tw_src_impl<float>(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, f_t12, f_t12_o, p, pold, rho, u, v, w, const_coeff);
+}
+
+
+int flutas_pres_sp_src(int nx, int ny, int nz, double f_t12, double dxi,
+                       double dyi, double dzi, int nh_d, int nh_u,
+                       const double *dzci, double rho0i, const double *pold,
+                       double *u, double *v, double *w) {
+  return pres_sp_src_impl<double>(nx, ny, nz, f_t12, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, pold, u, v, w);
+}
+int flutas_pres_sp_src_f32(int nx, int ny, int nz, float f_t12, float dxi,
+                       float dyi, float dzi, int nh_d, int nh_u,
+                       const float *dzci, float rho0i, const float *pold,
+                       float *u, float *v, float *w) {
+  return pres_sp_src_impl<float>(nx, ny, nz, f_t12, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, pold, u, v, w);
+}
+
+
+int flutas_correc(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
+                  double dyi, double dzi, const double *dzci, double dt,
+                  double rho0, const double *p, double *u, double *v,
+                  double *w, const double *rho, int const_coeff) {
+  return correc_impl<double>(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, p, u, v, w, rho, const_coeff);
+}
+int flutas_correc_f32(int nx, int ny, int nz, int nh_d, int nh_u, float dxi,
+                  float dyi, float dzi, const float *dzci, float dt,
+                  float rho0, const float *p, float *u, float *v,
+                  float *w, const float *rho, int const_coeff) {
+  return correc_impl<float>(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, p, u, v, w, rho, const_coeff);
+}
+
 
 int flutas_chkdiv(int nx, int ny, int nz, double dxi, double dyi, double dzi,
                   int nh_d, int nh_u, const double *dzfi, const double *u,
                   const double *v, const double *w, double *divtot,
                   double *divmax) {
-  (void)dzi;
-  FL_TRY(check_dims(nx, ny, nz));
-  if (!dzfi || !u || !v || !w || !divtot || !divmax) return fail(FLUTAS_EINVAL, "null pointer");
-  hipStream_t s = current_stream();
-  const int nb = chkdiv_blocks((long long)nx * ny * nz);
-  double *work = nullptr;
-  FL_HIP(hipMallocAsync((void **)&work, sizeof(double) * (2 * nb + 2), s));
-  double res[2] = {0.0, 0.0};
-  const int rc = chkdiv_enqueue(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzfi, u, v, w, work, res, s);
-  FL_HIP(hipFreeAsync(work, s));
-  FL_HIP(hipStreamSynchronize(s));
-  if (rc) return rc;
+  double res[2];
+  FL_TRY(chkdiv_impl<double>(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w, res));
   *divtot = res[0];
   *divmax = res[1];
+  return FLUTAS_OK;
+}
+
+int flutas_chkdiv_f32(int nx, int ny, int nz, float dxi, float dyi, float dzi,
+                      int nh_d, int nh_u, const float *dzfi, const float *u,
+                      const float *v, const float *w, float *divtot,
+                      float *divmax) {
+  double res[2];
+  if (!divtot || !divmax) return fail(FLUTAS_EINVAL, "null pointer");
+  FL_TRY(chkdiv_impl<float>(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w, res));
+  *divtot = (float)res[0];
+  *divmax = (float)res[1];
   return FLUTAS_OK;
 }
 
@@ -1009,32 +1309,13 @@ int flutas_chkdiv_noalloc(int nx, int ny, int nz, double dxi, double dyi, double
 
 int flutas_boundp(const char cbc[6], const int n[3], const double bc[6],
                   const double dl[3], double dzc_0, double dzc_n, double *p) {
-  if (!cbc || !n || !bc || !dl || !p) return fail(FLUTAS_EINVAL, "null pointer");
-  FL_TRY(check_dims(n[0], n[1], n[2]));
-  for (int q = 0; q < 6; ++q)
-    if (cbc[q] != 'P' && cbc[q] != 'D' && cbc[q] != 'N')
-      return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
-  hipStream_t s = current_stream();
-  const int nx = n[0], ny = n[1], nz = n[2];
-  auto face = [&](char t, int side, int dir, double value, double dr) -> int {
-    const int ext[3] = {nx + 2, ny + 2, nz + 2};
-    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
-    dim3 grid((ext[t1] + 255) / 256, ext[t2]);
-    hipLaunchKernelGGL(k_set_bc, grid, dim3(256), 0, s, nx, ny, nz, t, side,
-                       dir, value, dr, p);
-    FL_HIP(hipGetLastError());
-    return FLUTAS_OK;
-  };
-  // periodic y and z halos first (the self-exchange of updthalo at one
-  // rank, src/bound.f90:182-183), then set_bc for x, y, z
-  if (cbc[2] == 'P') { FL_TRY(face('P', 0, 1, 0, 0)); FL_TRY(face('P', 1, 1, 0, 0)); }
-  if (cbc[4] == 'P') { FL_TRY(face('P', 0, 2, 0, 0)); FL_TRY(face('P', 1, 2, 0, 0)); }
-  FL_TRY(face(cbc[0], 0, 0, bc[0], dl[0]));
-  FL_TRY(face(cbc[1], 1, 0, bc[1], dl[0]));
-  if (cbc[2] != 'P') { FL_TRY(face(cbc[2], 0, 1, bc[2], dl[1])); FL_TRY(face(cbc[3], 1, 1, bc[3], dl[1])); }
-  if (cbc[4] != 'P') { FL_TRY(face(cbc[4], 0, 2, bc[4], dzc_0)); FL_TRY(face(cbc[5], 1, 2, bc[5], dzc_n)); }
-  return FLUTAS_OK;
+  return boundp_impl<double>(cbc, n, bc, dl, dzc_0, dzc_n, p);
 }
+int flutas_boundp_f32(const char cbc[6], const int n[3], const float bc[6],
+                  const float dl[3], float dzc_0, float dzc_n, float *p) {
+  return boundp_impl<float>(cbc, n, bc, dl, dzc_0, dzc_n, p);
+}
+
 
 // bounduvw at one rank, in the decomposition model of flutas_boundp:
 // periodic y and z halos (updthalo's self-exchange) first, then the x faces
@@ -1046,182 +1327,41 @@ int flutas_bounduvw(const char cbc[18], const int n[3], const double bc[18],
                     int nh_d, int nh_u, const int isoutflow[6],
                     const double dl[3], const double *dzc, const double *dzf,
                     double *u, double *v, double *w) {
-  if (!cbc || !n || !bc || !isoutflow || !dl || !dzc || !dzf || !u || !v || !w)
-    return fail(FLUTAS_EINVAL, "null pointer");
-  FL_TRY(check_dims(n[0], n[1], n[2]));
-  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
-  if (nh_u < 1 || nh_u > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_u must be in 1..8");
-  if (nh_d < nh_u) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_u (dzc, dzf at the halo layers)");
-  for (int d = 0; d < 3; ++d)
-    if (n[d] + 2 * nh_u > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
-  auto CB = [&](int s, int d, int c) { return cbc[s + 2 * d + 6 * c]; };
-  auto BV = [&](int s, int d, int c) { return bc[s + 2 * d + 6 * c]; };
-  for (int c = 0; c < 3; ++c)
-    for (int d = 0; d < 3; ++d) {
-      for (int s = 0; s < 2; ++s)
-        if (CB(s, d, c) != 'P' && CB(s, d, c) != 'D' && CB(s, d, c) != 'N')
-          return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
-      if ((CB(0, d, c) == 'P') != (CB(1, d, c) == 'P'))
-        return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
-    }
-  hipStream_t s = current_stream();
-  double *fld[3] = {u, v, w};
-  const int nz = n[2];
-  auto Z = [&](const double *a, int k) { return a[k + nh_d - 1]; };
-  auto launch = [&](FaceArgs &a) -> int {
-    const int t1 = (a.dir == 0) ? 1 : 0, t2 = (a.dir == 2) ? 1 : 2;
-    const int e1 = n[t1] + 2 * nh_u, e2 = n[t2] + 2 * nh_u;
-    hipLaunchKernelGGL(k_set_bc_face, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
-    FL_HIP(hipGetLastError());
-    return FLUTAS_OK;
-  };
-  auto face = [&](int dir, int side) {
-    FaceArgs a{};
-    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
-    a.nh = nh_u;
-    a.dir = dir;
-    a.side = side;
-    return a;
-  };
-  // updthalo in y and z (periodic directions)
-  for (int d = 1; d <= 2; ++d) {
-    FaceArgs a = face(d, 0);
-    for (int c = 0; c < 3; ++c)
-      if (CB(0, d, c) == 'P') {
-        a.f[c].p = fld[c];
-        a.f[c].ctype = 'P';
-        a.f[c].centered = 1;
-      }
-    if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
-  }
-  for (int d = 0; d < 3; ++d)
-    for (int side = 0; side < 2; ++side) {
-      FaceArgs a = face(d, side);
-      for (int c = 0; c < 3; ++c) {
-        if (d > 0 && CB(side, d, c) == 'P') continue;  // done by updthalo
-        // a periodic set_bc copies both sides; its second call (side 1)
-        // would repeat the same copies
-        if (d == 0 && side == 1 && CB(side, d, c) == 'P') continue;
-        FaceField &F = a.f[c];
-        F.p = fld[c];
-        F.ctype = CB(side, d, c);
-        F.centered = (c != d);
-        F.value = BV(side, d, c);
-        for (int q = 0; q < nh_u; ++q) {
-          if (d < 2) F.dr[q] = dl[d];
-          else if (c < 2) F.dr[q] = (side == 0) ? Z(dzc, -q) : Z(dzc, nz + q);
-          else F.dr[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
-        }
-      }
-      if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
-    }
-  // outflow faces
-  bool need_dzf = false;
-  for (int d = 0; d < 2; ++d) need_dzf = need_dzf || isoutflow[2 * d] || isoutflow[2 * d + 1];
-  double *dzf_d = nullptr;
-  const size_t nzh = (size_t)nz + 2 * nh_d;
-  if (need_dzf) {
-    FL_HIP(hipMalloc((void **)&dzf_d, sizeof(double) * nzh));
-    hipError_t e = hipMemcpy(dzf_d, dzf, sizeof(double) * nzh, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(dzf_d);
-      return fail(FLUTAS_EDEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    }
-  }
-  int rc = FLUTAS_OK;
-  for (int d = 0; d < 3 && rc == FLUTAS_OK; ++d)
-    for (int side = 0; side < 2 && rc == FLUTAS_OK; ++side) {
-      if (!isoutflow[side + 2 * d]) continue;
-      OutflowArgs a{};
-      a.u = u; a.v = v; a.w = w;
-      a.dzf = dzf_d;
-      // layers written: 0..nh_u on the upper side, 0..nh_u-1 on the lower
-      for (int q = 0; q <= (side == 1 ? nh_u : nh_u - 1); ++q)
-        a.dzq[q] = (side == 0) ? Z(dzf, -q) : Z(dzf, nz + q);
-      a.dx = dl[0]; a.dy = dl[1];
-      a.dxi = 1.0 / dl[0]; a.dyi = 1.0 / dl[1];
-      for (int q = 0; q < 3; ++q) a.n[q] = n[q];
-      a.nh = nh_u; a.nh_d = nh_d; a.dir = d; a.side = side;
-      const int t1 = (d == 0) ? n[1] : n[0], t2 = (d == 2) ? n[1] : n[2];
-      hipLaunchKernelGGL(k_outflow, dim3((t1 + 255) / 256, t2), dim3(256), 0, s, a);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess) rc = fail(FLUTAS_EDEVICE, std::string("k_outflow: ") + hipGetErrorString(e));
-    }
-  if (dzf_d) {
-    hipError_t e = hipStreamSynchronize(s);
-    (void)hipFree(dzf_d);
-    if (rc == FLUTAS_OK && e != hipSuccess) rc = fail(FLUTAS_EDEVICE, hipGetErrorString(e));
-  }
-  return rc;
+  return bounduvw_impl<double>(cbc, n, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w);
 }
+int flutas_bounduvw_f32(const char cbc[18], const int n[3], const float bc[18],
+                    int nh_d, int nh_u, const int isoutflow[6],
+                    const float dl[3], const float *dzc, const float *dzf,
+                    float *u, float *v, float *w) {
+  return bounduvw_impl<float>(cbc, n, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w);
+}
+
 
 // boundp for a field with halo nh_p (src/bound.f90:146-225): the face
 // launches of flutas_bounduvw on one cell-centred field.
 int flutas_boundp_halo(const char cbc[6], const int n[3], const double bc[6],
                        int nh_d, int nh_p, const double dl[3],
                        const double *dzc, const double *dzf, double *p) {
-  (void)dzf;
-  if (!cbc || !n || !bc || !dl || !dzc || !p) return fail(FLUTAS_EINVAL, "null pointer");
-  FL_TRY(check_dims(n[0], n[1], n[2]));
-  if (n[0] < 2 || n[1] < 2 || n[2] < 2) return fail(FLUTAS_EINVAL, "grid sizes must be >= 2");
-  if (nh_p < 1 || nh_p > kMaxHalo) return fail(FLUTAS_EINVAL, "nh_p must be in 1..8");
-  if (nh_d < nh_p) return fail(FLUTAS_EINVAL, "nh_d must be >= nh_p (dzc at the halo layers)");
-  for (int d = 0; d < 3; ++d) {
-    if (n[d] + 2 * nh_p > 65535) return fail(FLUTAS_EINVAL, "grid too large for the face launches");
-    for (int s = 0; s < 2; ++s)
-      if (cbc[2 * d + s] != 'P' && cbc[2 * d + s] != 'D' && cbc[2 * d + s] != 'N')
-        return fail(FLUTAS_EINVAL, "BC must be one of P, D, N");
-    if ((cbc[2 * d] == 'P') != (cbc[2 * d + 1] == 'P'))
-      return fail(FLUTAS_EINVAL, "periodic BC must be set on both sides");
-  }
-  hipStream_t s = current_stream();
-  const int nz = n[2];
-  auto run = [&](int dir, int side, char t, double value, double dr0, bool zlo, bool zhi) -> int {
-    FaceArgs a{};
-    for (int q = 0; q < 3; ++q) a.n[q] = n[q];
-    a.nh = nh_p;
-    a.dir = dir;
-    a.side = side;
-    FaceField &F = a.f[0];
-    F.p = p;
-    F.ctype = t;
-    F.centered = 1;
-    F.value = value;
-    for (int q = 0; q < nh_p; ++q)
-      F.dr[q] = zlo ? dzc[-q + nh_d - 1] : (zhi ? dzc[nz + q + nh_d - 1] : dr0);
-    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
-    const int e1 = n[t1] + 2 * nh_p, e2 = n[t2] + 2 * nh_p;
-    hipLaunchKernelGGL(k_set_bc_face, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
-    FL_HIP(hipGetLastError());
-    return FLUTAS_OK;
-  };
-  // updthalo in y and z (periodic), then x, y, z faces as set_bc
-  if (cbc[2] == 'P') FL_TRY(run(1, 0, 'P', 0.0, 0.0, false, false));
-  if (cbc[4] == 'P') FL_TRY(run(2, 0, 'P', 0.0, 0.0, false, false));
-  FL_TRY(run(0, 0, cbc[0], bc[0], dl[0], false, false));
-  if (cbc[1] != 'P') FL_TRY(run(0, 1, cbc[1], bc[1], dl[0], false, false));  // P: both sides done
-  if (cbc[2] != 'P') FL_TRY(run(1, 0, cbc[2], bc[2], dl[1], false, false));
-  if (cbc[3] != 'P') FL_TRY(run(1, 1, cbc[3], bc[3], dl[1], false, false));
-  if (cbc[4] != 'P') FL_TRY(run(2, 0, cbc[4], bc[4], 0.0, true, false));
-  if (cbc[5] != 'P') FL_TRY(run(2, 1, cbc[5], bc[5], 0.0, false, true));
-  return FLUTAS_OK;
+  return boundp_halo_impl<double>(cbc, n, bc, nh_d, nh_p, dl, dzc, dzf, p);
 }
+int flutas_boundp_halo_f32(const char cbc[6], const int n[3], const float bc[6],
+                       int nh_d, int nh_p, const float dl[3],
+                       const float *dzc, const float *dzf, float *p) {
+  return boundp_halo_impl<float>(cbc, n, bc, nh_d, nh_p, dl, dzc, dzf, p);
+}
+
 
 int flutas_updt_rhs_b(int nx, int ny, int nz, const char c_or_f[3],
                       const char cbc[6], const double *rhsbx,
                       const double *rhsby, const double *rhsbz, double *p) {
-  FL_TRY(check_dims(nx, ny, nz));
-  if (!c_or_f || !cbc || !rhsbx || !rhsby || !rhsbz || !p) return fail(FLUTAS_EINVAL, "null pointer");
-  const int q1 = (c_or_f[0] == 'f' && cbc[1] == 'D') ? 1 : 0;
-  const int q2 = (c_or_f[1] == 'f' && cbc[3] == 'D') ? 1 : 0;
-  const int q3 = (c_or_f[2] == 'f' && cbc[5] == 'D') ? 1 : 0;
-  hipStream_t s = current_stream();
-  hipLaunchKernelGGL(k_add_face, dim3((ny + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 0, q1, rhsbx, p);
-  hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, nz), dim3(256), 0, s, nx, ny, nz, 1, q2, rhsby, p);
-  hipLaunchKernelGGL(k_add_face, dim3((nx + 255) / 256, ny), dim3(256), 0, s, nx, ny, nz, 2, q3, rhsbz, p);
-  FL_HIP(hipGetLastError());
-  return FLUTAS_OK;
+  return updt_rhs_b_impl<double>(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p);
 }
+int flutas_updt_rhs_b_f32(int nx, int ny, int nz, const char c_or_f[3],
+                      const char cbc[6], const float *rhsbx,
+                      const float *rhsby, const float *rhsbz, float *p) {
+  return updt_rhs_b_impl<float>(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p);
+}
+
 
 int flutas_fillps_rhs(int nx, int ny, int nz, int nh_d, int nh_u, double dxi,
                       double dyi, double dzi, const double *dzfi, double dti,

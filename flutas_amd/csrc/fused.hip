@@ -279,6 +279,7 @@ template int launch_x_fwd<double, double>(const LineTables &, int, const double 
 template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_bwd<double, double>(const LineTables &, int, const double *, WStride, double *, long long, long long, int, int, double, int, hipStream_t);
+template int launch_x_bwd<float, float>(const LineTables &, int, const float *, WStride, float *, long long, long long, int, int, float, int, hipStream_t);
 template int launch_x_bwd<float, double>(const LineTables &, int, const float *, WStride, double *, long long, long long, int, int, float, int, hipStream_t);
 template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
 template int launch_yz_fwd<float>(const LineTables &, int, float *, WStride, double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);

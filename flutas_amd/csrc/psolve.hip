@@ -285,8 +285,8 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
 // -> out (interior origin of a halo-1 FP64 field): out = S(in), or
 // out += S(in) when accumulate.  The fused FP64 passes keep W in blocks
 // of x-slabs (fused_layout_fp64), the FP32 ones plane-major.
-template <typename T, typename Tin>
-int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
+template <typename T, typename Tin, typename Tout = double>
+int spectral(flutas_psolve *h, const Tin *in, bool in_halo, Tout *out,
              int accumulate, T *W, hipStream_t s, double *absmax_part = nullptr,
              int *nf_flag = nullptr) {
   const Geo q = halo_geo(h);
@@ -307,8 +307,8 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
     FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
                             h->d_lamy, h->d_a, h->d_b, h->d_c, per, s));
     FL_TRY(mark(h, "P3_zy_bwd", s));
-    FL_TRY((launch_x_bwd<T, double>(h->tx, h->kinds[1], W, ws, out, q.hx, q.hxy, ny, nz,
-                                    (T)h->normfft, accumulate, s)));
+    FL_TRY((launch_x_bwd<T, Tout>(h->tx, h->kinds[1], W, ws, out, q.hx, q.hxy, ny, nz,
+                                  (T)h->normfft, accumulate, s)));
     FL_TRY(mark(h, "P4_x_bwd", s));
     return FLUTAS_OK;
   }
@@ -324,13 +324,15 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, double *out,
   FL_TRY((launch_lines<T, T, T>(h->ty, h->kinds[3], W, W, geom_y(h), T(1), s)));
   FL_TRY(mark(h, "y_bwd", s));
   if (!accumulate) {
-    FL_TRY((launch_lines<T, T, double>(h->tx, h->kinds[1], W, out, geom_x_to_halo(h),
-                                       (T)h->normfft, s)));
-  } else {
+    FL_TRY((launch_lines<T, T, Tout>(h->tx, h->kinds[1], W, out, geom_x_to_halo(h),
+                                     (T)h->normfft, s)));
+  } else if constexpr (sizeof(Tout) == 8) {
     // only the refined mode accumulates: through the dense FP32 residual buffer
     FL_TRY((launch_lines<T, T, T>(h->tx, h->kinds[1], W, (T *)h->R32, geom_x_dense(h),
                                   (T)h->normfft, s)));
     FL_TRY(launch_accumulate(h->R32, out - q.interior, nx, ny, nz, s));
+  } else {
+    return fail(FLUTAS_EINVAL, "no accumulation into an FP32 field");
   }
   FL_TRY(mark(h, "x_bwd", s));
   return FLUTAS_OK;
@@ -342,6 +344,26 @@ int solve_fp64(flutas_psolve *h, const double *f, double *p, hipStream_t s) {
   if (!h->fused) FL_TRY(launch_flag_nonfinite(f, h->nx, h->ny, h->nz, h->nf_dev, h->seq, s));
   FL_TRY((spectral<double, double>(h, f + q.interior, true, p + q.interior, 0, h->W64, s,
                                    nullptr, h->fused ? h->nf_dev : nullptr)));
+  h->last_iters = 0;
+  h->last_relres = -1.0;
+  return FLUTAS_OK;
+}
+
+// The solve of an FP32 field (a FluTAS build with _SINGLE_PRECISION, rp =
+// FP32: src/types.f90:6-12), in place or out of place: the FP32 chain of
+// the refined mode (FP32 transforms, FP64 z elimination) once, no
+// refinement, as the reference's single-precision solver runs its whole
+// solve in FP32.  The FP32 work field is allocated on first use when the
+// plan is an FP64 one.
+int solve_f32(flutas_psolve *h, const float *f, float *p, hipStream_t s) {
+  if (!h->W32) {
+    const size_t N = (size_t)h->nx * h->ny * h->nz;
+    FL_HIP(hipMalloc((void **)&h->W32, N * sizeof(float)));
+  }
+  const Geo q = halo_geo(h);
+  ++h->seq;
+  FL_TRY((spectral<float, float, float>(h, f + q.interior, true, p + q.interior, 0, h->W32, s,
+                                        nullptr, h->fused ? h->nf_dev : nullptr)));
   h->last_iters = 0;
   h->last_relres = -1.0;
   return FLUTAS_OK;
@@ -669,6 +691,16 @@ int flutas_psolve_solve_into(flutas_psolve *h, const double *f, double *p) {
   return check_nonfinite(h);
 }
 
+int flutas_psolve_solve_f32(flutas_psolve *h, float *p) {
+  if (!h || !p) return fail(FLUTAS_EINVAL, "null pointer");
+  hipStream_t s = current_stream();
+  FL_TRY(solve_f32(h, p, p, s));
+  FL_HIP(hipStreamSynchronize(s));
+  if (h->fused && *(volatile int *)h->nf_host == h->seq)
+    return fail(FLUTAS_ENONFINITE, "right-hand side holds NaN or Inf");
+  return FLUTAS_OK;
+}
+
 int flutas_psolve_solve(flutas_psolve *h, double *p) {
   FL_TRY(flutas_psolve_solve_async(h, p));
   FL_HIP(hipStreamSynchronize(current_stream()));
@@ -805,8 +837,14 @@ int flutas_fft_lines(int kind, int n, int howmany, long long stride,
   return flutas_fft_guru(kind, n, stride, howmany, dist, 1, 0, data);
 }
 
-int flutas_fft_guru(int kind, int n, long long stride, int howmany1, long long dist1,
-                    int howmany2, long long dist2, double *data) {
+}  // extern "C"
+
+// fft of fftini's line geometries (src/fft.f90:181-193) on FP64 or FP32
+// data (the FP32 variant serves FluTAS built with _SINGLE_PRECISION, whose
+// FFTW plans are single precision).
+template <typename T>
+static int fft_guru_impl(int kind, int n, long long stride, int howmany1, long long dist1,
+                         int howmany2, long long dist2, T *data) {
   if (!data || howmany1 < 1 || howmany2 < 1 || howmany2 > 65535 || stride < 1 || n < 2)
     return fail(FLUTAS_EINVAL, "bad fft arguments");
   if (kind != K_R2HC && kind != K_HC2R && kind != K_REDFT01 && kind != K_REDFT10 &&
@@ -823,12 +861,24 @@ int flutas_fft_guru(int kind, int n, long long stride, int howmany1, long long d
   g.n = n; g.L1 = howmany1; g.L2 = howmany2;
   g.is1 = dist1; g.is2 = dist2; g.ise = stride;
   g.os1 = dist1; g.os2 = dist2; g.ose = stride;
-  int rc = launch_lines<double, double, double>(tab, kind, data, data, g, 1.0, current_stream());
+  int rc = launch_lines<T, T, T>(tab, kind, data, data, g, T(1), current_stream());
   hipError_t e = hipStreamSynchronize(current_stream());
   tab.release();
   if (rc) return rc;
   if (e != hipSuccess) return fail(FLUTAS_EDEVICE, hipGetErrorString(e));
   return FLUTAS_OK;
+}
+
+extern "C" {
+
+int flutas_fft_guru(int kind, int n, long long stride, int howmany1, long long dist1,
+                    int howmany2, long long dist2, double *data) {
+  return fft_guru_impl<double>(kind, n, stride, howmany1, dist1, howmany2, dist2, data);
+}
+
+int flutas_fft_guru_f32(int kind, int n, long long stride, int howmany1, long long dist1,
+                        int howmany2, long long dist2, float *data) {
+  return fft_guru_impl<float>(kind, n, stride, howmany1, dist1, howmany2, dist2, data);
 }
 
 int flutas_event_create(void **ev) {

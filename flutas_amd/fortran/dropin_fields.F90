@@ -1,6 +1,8 @@
 ! Drop-in field modules of the projection step: the reference's fillps,
 ! correc, pres_tw_src, pres_sp_src, chkdiv, boundp and updt_rhs_b
-! signatures, backed by the flutas_amd field kernels.
+! signatures, backed by the flutas_amd field kernels.  Fields are real(rp)
+! (flutas_kinds): FP64, or FP32 in a _SINGLE_PRECISION build, which calls
+! the library's FP32 entry points.
 !
 ! Fields that are HIP device or managed allocations go to the kernels as
 ! they are; plain host arrays are staged through device buffers
@@ -9,6 +11,7 @@
 ! pres_tw_src.
 module flutas_field_sizes
   use, intrinsic :: iso_c_binding
+  use flutas_kinds, only: rp_bytes
   implicit none
   private
   public :: nbytes
@@ -16,7 +19,7 @@ contains
   pure function nbytes(nelem) result(b)
     integer, intent(in) :: nelem
     integer(c_size_t) :: b
-    b = 8_c_size_t * int(nelem, c_size_t)
+    b = rp_bytes * int(nelem, c_size_t)
   end function nbytes
 end module flutas_field_sizes
 !
@@ -26,10 +29,10 @@ module mod_fillps
   use flutas_capi
   use flutas_stage
   use flutas_field_sizes
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: fillps
-  integer, parameter :: rp = c_double
 contains
   subroutine fillps(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dzfi,dti,rho0,u,v,w,p)
     integer , intent(in )                                     :: nx,ny,nz
@@ -54,7 +57,11 @@ contains
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dp = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
     dz = stage_in(5, c_loc(dzfi), nbytes(size(dzfi)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_fillps_f32(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dti,scale,du,dv,dw,dp), 'fillps')
+#else
     call flutas_check(flutas_fillps(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dti,scale,du,dv,dw,dp), 'fillps')
+#endif
     call stage_out(c_loc(p), dp, nbytes(size(p)))
   end subroutine fillps
 end module mod_fillps
@@ -65,10 +72,10 @@ module mod_correc
   use flutas_capi
   use flutas_stage
   use flutas_field_sizes
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: correc
-  integer, parameter :: rp = c_double
 contains
   subroutine correc(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dzci,dt,rho0,p,u,v,w,rho)
     integer , intent(in   )                                     :: nx,ny,nz
@@ -93,7 +100,11 @@ contains
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dp = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
     dz = stage_in(6, c_loc(dzci), nbytes(size(dzci)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_correc_f32(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dt,rho0,dp,du,dv,dw,dr,cc), 'correc')
+#else
     call flutas_check(flutas_correc(nx,ny,nz,nh_d,nh_u,dxi,dyi,dzi,dz,dt,rho0,dp,du,dv,dw,dr,cc), 'correc')
+#endif
     call stage_out(c_loc(u), du, nbytes(size(u)))
     call stage_out(c_loc(v), dv, nbytes(size(v)))
     call stage_out(c_loc(w), dw, nbytes(size(w)))
@@ -106,10 +117,10 @@ module mod_source
   use flutas_capi
   use flutas_stage
   use flutas_field_sizes
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: pres_tw_src, pres_sp_src
-  integer, parameter :: rp = c_double
 contains
   subroutine pres_tw_src(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dzci,rho0i,f_t12,f_t12_o,p,pold,rho,u,v,w)
     integer , intent(in   )                                     :: nx,ny,nz
@@ -134,8 +145,13 @@ contains
     dpo = stage_in(5, c_loc(pold), nbytes(size(pold)), .true.)
     dr  = stage_in(6, c_loc(rho), nbytes(size(rho)), .true.)
     dz  = stage_in(7, c_loc(dzci), nbytes(size(dzci)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_pres_tw_src_f32(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dz,rho0i,f_t12,f_t12_o, &
+                                         dp,dpo,dr,du,dv,dw,cc), 'pres_tw_src')
+#else
     call flutas_check(flutas_pres_tw_src(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dz,rho0i,f_t12,f_t12_o, &
                                          dp,dpo,dr,du,dv,dw,cc), 'pres_tw_src')
+#endif
     call stage_out(c_loc(u), du, nbytes(size(u)))
     call stage_out(c_loc(v), dv, nbytes(size(v)))
     call stage_out(c_loc(w), dw, nbytes(size(w)))
@@ -156,8 +172,13 @@ contains
     dw  = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dpo = stage_in(5, c_loc(pold), nbytes(size(pold)), .true.)
     dz  = stage_in(7, c_loc(dzci), nbytes(size(dzci)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_pres_sp_src_f32(nx,ny,nz,f_t12,dxi,dyi,dzi,nh_d,nh_u,dz,rho0i,dpo,du,dv,dw), &
+                      'pres_sp_src')
+#else
     call flutas_check(flutas_pres_sp_src(nx,ny,nz,f_t12,dxi,dyi,dzi,nh_d,nh_u,dz,rho0i,dpo,du,dv,dw), &
                       'pres_sp_src')
+#endif
     call stage_out(c_loc(u), du, nbytes(size(u)))
     call stage_out(c_loc(v), dv, nbytes(size(v)))
     call stage_out(c_loc(w), dw, nbytes(size(w)))
@@ -170,10 +191,10 @@ module mod_chkdiv
   use flutas_capi
   use flutas_stage
   use flutas_field_sizes
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: chkdiv
-  integer, parameter :: rp = c_double
 contains
   subroutine chkdiv(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dzfi,u,v,w,divtot,divmax)
     integer , intent(in )                                     :: nx,ny,nz
@@ -187,7 +208,11 @@ contains
     dv = stage_in(2, c_loc(v), nbytes(size(v)), .true.)
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
     dz = stage_in(5, c_loc(dzfi), nbytes(size(dzfi)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_chkdiv_f32(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dz,du,dv,dw,divtot,divmax), 'chkdiv')
+#else
     call flutas_check(flutas_chkdiv(nx,ny,nz,dxi,dyi,dzi,nh_d,nh_u,dz,du,dv,dw,divtot,divmax), 'chkdiv')
+#endif
     ! the report the reference prints (src/chkdiv.f90:66; one rank)
     print*, 'Total divergence = ', divtot, '| Maximum divergence = ', divmax
   end subroutine chkdiv
@@ -201,10 +226,10 @@ module mod_bound
   use flutas_capi
   use flutas_stage
   use flutas_field_sizes
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: bounduvw, boundp, updt_rhs_b
-  integer, parameter :: rp = c_double
 contains
   ! host copy of a metric array that may live in device memory
   subroutine metric_to_host(arr, out)
@@ -230,7 +255,7 @@ contains
     real(rp)        , intent(in   ), dimension(1-nh_d:), target, contiguous :: dzc,dzf
     real(rp)        , intent(inout), dimension(1-nh_u:,1-nh_u:,1-nh_u:), target, contiguous :: u,v,w
     character(kind=c_char) :: cbc_c(18)
-    real(c_double) :: bc_c(18)
+    real(rp) :: bc_c(18)
     integer(c_int) :: nn(3), out_c(6)
     real(rp), allocatable, target :: hdzc(:), hdzf(:)
     type(c_ptr) :: du, dv, dw
@@ -256,8 +281,13 @@ contains
     du = stage_in(1, c_loc(u), nbytes(size(u)), .true.)
     dv = stage_in(2, c_loc(v), nbytes(size(v)), .true.)
     dw = stage_in(3, c_loc(w), nbytes(size(w)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_bounduvw_f32(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_u, c_int), out_c, &
+                                      dl, hdzc, hdzf, du, dv, dw), 'bounduvw')
+#else
     call flutas_check(flutas_bounduvw(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_u, c_int), out_c, &
                                       dl, hdzc, hdzf, du, dv, dw), 'bounduvw')
+#endif
     call stage_out(c_loc(u), du, nbytes(size(u)))
     call stage_out(c_loc(v), dv, nbytes(size(v)))
     call stage_out(c_loc(w), dw, nbytes(size(w)))
@@ -273,7 +303,7 @@ contains
     real(rp)        , intent(in   ), dimension(1-nh_d:), target, contiguous :: dzc,dzf
     real(rp)        , intent(inout), dimension(1-nh_p:,1-nh_p:,1-nh_p:), target, contiguous :: p
     character(kind=c_char) :: cbc_c(6)
-    real(c_double) :: bc_c(6)
+    real(rp) :: bc_c(6)
     integer(c_int) :: nn(3)
     real(rp), allocatable, target :: hdzc(:), hdzf(:)
     type(c_ptr) :: dp
@@ -290,8 +320,13 @@ contains
     call metric_to_host(dzc, hdzc)
     call metric_to_host(dzf, hdzf)
     dp = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_boundp_halo_f32(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_p, c_int), dl, &
+                                         hdzc, hdzf, dp), 'boundp')
+#else
     call flutas_check(flutas_boundp_halo(cbc_c, nn, bc_c, int(nh_d, c_int), int(nh_p, c_int), dl, &
                                          hdzc, hdzf, dp), 'boundp')
+#endif
     call stage_out(c_loc(p), dp, nbytes(size(p)))
   end subroutine boundp
   !
@@ -317,7 +352,11 @@ contains
     dby = stage_in(6, c_loc(rhsby), nbytes(size(rhsby)), .true.)
     dbz = stage_in(7, c_loc(rhsbz), nbytes(size(rhsbz)), .true.)
     dp  = stage_in(4, c_loc(p), nbytes(size(p)), .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_updt_rhs_b_f32(nx,ny,nz,cf_c,cbc_c,dbx,dby,dbz,dp), 'updt_rhs_b')
+#else
     call flutas_check(flutas_updt_rhs_b(nx,ny,nz,cf_c,cbc_c,dbx,dby,dbz,dp), 'updt_rhs_b')
+#endif
     call stage_out(c_loc(p), dp, nbytes(size(p)))
   end subroutine updt_rhs_b
 end module mod_bound

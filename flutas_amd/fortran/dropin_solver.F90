@@ -17,14 +17,17 @@
 !
 ! flutas_set_precision(mode, tol), called before initsolver, selects
 ! FLUTAS_FP64 (default) or FLUTAS_FP32_REFINE with a relative-residual
-! target.
+! target.  Reals are real(rp) (flutas_kinds): in a _SINGLE_PRECISION build
+! the metric arrays go to the plan as FP64 copies, its coefficients come
+! back rounded to FP32, and the solves and transforms run the library's
+! FP32 entry points (FP32 transforms, FP64 z elimination).
 module mod_initsolver
   use, intrinsic :: iso_c_binding
   use flutas_capi
+  use flutas_kinds, only: rp
   implicit none
   private
   public :: initsolver, flutas_set_precision
-  integer, parameter :: rp = c_double
   integer(c_int), save :: precision_mode = FLUTAS_FP64
   real(rp), save :: refine_tol = 1.0e-10_rp
 contains
@@ -54,7 +57,9 @@ contains
     !
     integer(c_int) :: ng(3)
     character(kind=c_char) :: cbc_c(6), cf_c(3)
-    real(c_double) :: bc_c(6)
+    real(c_double) :: bc_c(6), dli_c(3), nf_c
+    real(c_double), allocatable :: zci_c(:), zfi_c(:), lam_c(:,:), a_c(:), b_c(:), c_c(:), &
+                                   rx_c(:,:,:), ry_c(:,:,:), rz_c(:,:,:)
     type(c_ptr) :: h
     integer :: q, s, d
     if (any(dims /= 1) .or. any(dims_z /= 1)) &
@@ -68,11 +73,26 @@ contains
       end do
       cf_c(d) = c_or_f(d)
     end do
+    ! the plan takes and returns FP64 values (exact copies when rp is FP64)
+    dli_c = real(dli, c_double)
+    allocate(zci_c(size(dzci)), zfi_c(size(dzfi)))
+    zci_c = real(dzci, c_double)
+    zfi_c = real(dzfi, c_double)
+    allocate(lam_c(n(1),n(2)), a_c(n(3)), b_c(n(3)), c_c(n(3)), rx_c(n(2),n(3),0:1), &
+             ry_c(n(1),n(3),0:1), rz_c(n(1),n(2),0:1))
     h = c_null_ptr
-    call flutas_check(flutas_psolve_create(ng, cbc_c, bc_c, cf_c, dli, int(nh_d, c_int), dzci, dzfi, &
-                                           precision_mode, refine_tol, h), 'initsolver')
-    call flutas_check(flutas_psolve_coeffs(h, lambdaxy, a, b, c, normfft, rhsbx, rhsby, rhsbz), &
+    call flutas_check(flutas_psolve_create(ng, cbc_c, bc_c, cf_c, dli_c, int(nh_d, c_int), zci_c, zfi_c, &
+                                           precision_mode, real(refine_tol, c_double), h), 'initsolver')
+    call flutas_check(flutas_psolve_coeffs(h, lam_c, a_c, b_c, c_c, nf_c, rx_c, ry_c, rz_c), &
                       'initsolver coefficients')
+    lambdaxy = real(lam_c, rp)
+    a = real(a_c, rp)
+    b = real(b_c, rp)
+    c = real(c_c, rp)
+    normfft = real(nf_c, rp)
+    rhsbx = real(rx_c, rp)
+    rhsby = real(ry_c, rp)
+    rhsbz = real(rz_c, rp)
     arrplan(:,:) = c_null_ptr
     arrplan(1,1) = h
   end subroutine initsolver
@@ -82,10 +102,10 @@ module mod_fft
   use, intrinsic :: iso_c_binding
   use flutas_capi
   use flutas_stage, only: stage_in, stage_out, stage_release
+  use flutas_kinds, only: rp, rp_bytes
   implicit none
   private
   public :: fftini, fftend, fft
-  integer, parameter :: rp = c_double
   ! FFTW's r2r kind numbers (src/fftw.f90:38-61)
   integer(c_int), parameter :: R2HC = 0, HC2R = 1, REDFT00 = 3, REDFT01 = 4, REDFT10 = 5, &
                                REDFT11 = 6, RODFT00 = 7, RODFT01 = 8, RODFT10 = 9, RODFT11 = 10
@@ -207,10 +227,15 @@ contains
     if (plan_index(plan) == 0) error stop 'flutas_amd fft: plan not made by fftini'
     call c_f_pointer(plan, lp)
     if (size(arr, kind=c_size_t) /= lp%elems) error stop 'flutas_amd fft: array size differs from the plan'
-    bytes = 8_c_size_t*lp%elems
+    bytes = rp_bytes*lp%elems
     d = stage_in(6, c_loc(arr), bytes, .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_fft_guru_f32(lp%kind, lp%n, lp%stride, lp%howmany1, lp%dist1, &
+                                          lp%howmany2, lp%dist2, d), 'fft')
+#else
     call flutas_check(flutas_fft_guru(lp%kind, lp%n, lp%stride, lp%howmany1, lp%dist1, &
                                       lp%howmany2, lp%dist2, d), 'fft')
+#endif
     call stage_out(c_loc(arr), d, bytes)
   end subroutine fft
   !
@@ -255,7 +280,11 @@ contains
     type(c_ptr) :: dp
     if (.not. c_associated(h)) error stop 'flutas_amd solver: plan not initialised (initsolver)'
     dp = stage_in(4, p, nbytes, .true.)
+#if defined(_SINGLE_PRECISION)
+    call flutas_check(flutas_psolve_solve_f32(h, dp), 'solver')
+#else
     call flutas_check(flutas_psolve_solve(h, dp), 'solver')
+#endif
     call stage_out(p, dp, nbytes)
   end subroutine solve_in_place
 end module flutas_psolve_call
@@ -263,10 +292,10 @@ end module flutas_psolve_call
 module mod_solver_gpu
   use, intrinsic :: iso_c_binding
   use flutas_psolve_call
+  use flutas_kinds, only: rp, rp_bytes
   implicit none
   private
   public :: solver_gpu
-  integer, parameter :: rp = c_double
 contains
   subroutine solver_gpu(n,dims,arrplan,normfft,lambdaxy,a,b,c,bc,c_or_f,p)
     integer         , intent(in   ), dimension(3)     :: n
@@ -280,17 +309,17 @@ contains
     real(rp)        , intent(inout), dimension(0:,0:,0:), target, contiguous :: p
     if (any(dims /= 1)) error stop 'flutas_amd solver_gpu: one GPU per solve (dims must be 1)'
     if (any(shape(p) /= n + 2)) error stop 'flutas_amd solver_gpu: p must be (0:n+1)^3'
-    call solve_in_place(arrplan(1,1), c_loc(p), 8_c_size_t*size(p, kind=c_size_t))
+    call solve_in_place(arrplan(1,1), c_loc(p), rp_bytes*size(p, kind=c_size_t))
   end subroutine solver_gpu
 end module mod_solver_gpu
 !
 module mod_solver_cpu
   use, intrinsic :: iso_c_binding
   use flutas_psolve_call
+  use flutas_kinds, only: rp, rp_bytes
   implicit none
   private
   public :: solver_cpu
-  integer, parameter :: rp = c_double
 contains
   subroutine solver_cpu(n,arrplan,normfft,lambdaxy,a,b,c,bcz,c_or_f,p)
     integer         , intent(in   ), dimension(3)   :: n
@@ -302,6 +331,6 @@ contains
     character(len=1), intent(in   ), dimension(3)   :: c_or_f
     real(rp)        , intent(inout), dimension(0:,0:,0:), target, contiguous :: p
     if (any(shape(p) /= n + 2)) error stop 'flutas_amd solver_cpu: p must be (0:n+1)^3'
-    call solve_in_place(arrplan(1,1), c_loc(p), 8_c_size_t*size(p, kind=c_size_t))
+    call solve_in_place(arrplan(1,1), c_loc(p), rp_bytes*size(p, kind=c_size_t))
   end subroutine solver_cpu
 end module mod_solver_cpu

@@ -5,7 +5,7 @@
 module flutas_capi
   use, intrinsic :: iso_c_binding
   implicit none
-  private :: c_int, c_double, c_char, c_ptr, c_size_t, c_long_long
+  private :: c_int, c_double, c_float, c_char, c_ptr, c_size_t, c_long_long
   !
   integer(c_int), parameter :: FLUTAS_OK = 0, FLUTAS_FP64 = 64, FLUTAS_FP32_REFINE = 32
   !
@@ -236,6 +236,95 @@ module flutas_capi
       integer(c_int), value :: n, nh_d
       real(c_double), value :: gr, lz
       real(c_double), dimension(*) :: dzc, dzf, zc, zf
+      integer(c_int) :: rc
+    end function
+    ! FP32 fields and scalars (FluTAS built with _SINGLE_PRECISION)
+    function flutas_psolve_solve_f32(h, p) bind(C, name="flutas_psolve_solve_f32") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, p
+      integer(c_int) :: rc
+    end function
+    function flutas_fft_guru_f32(kind, n, stride, howmany1, dist1, howmany2, dist2, data) &
+        bind(C, name="flutas_fft_guru_f32") result(rc)
+      import :: c_int, c_long_long, c_ptr
+      integer(c_int), value :: kind, n, howmany1, howmany2
+      integer(c_long_long), value :: stride, dist1, dist2
+      type(c_ptr), value :: data
+      integer(c_int) :: rc
+    end function
+    function flutas_fillps_f32(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzfi, dti, rho0, u, v, w, p) &
+        bind(C, name="flutas_fillps_f32") result(rc)
+      import :: c_int, c_float, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u
+      real(c_float), value :: dxi, dyi, dzi, dti, rho0
+      type(c_ptr), value :: dzfi, u, v, w, p
+      integer(c_int) :: rc
+    end function
+    function flutas_pres_tw_src_f32(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, f_t12, f_t12_o, &
+        p, pold, rho, u, v, w, const_coeff) bind(C, name="flutas_pres_tw_src_f32") result(rc)
+      import :: c_int, c_float, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u, const_coeff
+      real(c_float), value :: dxi, dyi, dzi, rho0i, f_t12, f_t12_o
+      type(c_ptr), value :: dzci, p, pold, rho, u, v, w
+      integer(c_int) :: rc
+    end function
+    function flutas_pres_sp_src_f32(nx, ny, nz, f_t12, dxi, dyi, dzi, nh_d, nh_u, dzci, rho0i, pold, &
+        u, v, w) bind(C, name="flutas_pres_sp_src_f32") result(rc)
+      import :: c_int, c_float, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u
+      real(c_float), value :: f_t12, dxi, dyi, dzi, rho0i
+      type(c_ptr), value :: dzci, pold, u, v, w
+      integer(c_int) :: rc
+    end function
+    function flutas_correc_f32(nx, ny, nz, nh_d, nh_u, dxi, dyi, dzi, dzci, dt, rho0, p, u, v, w, &
+        rho, const_coeff) bind(C, name="flutas_correc_f32") result(rc)
+      import :: c_int, c_float, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u, const_coeff
+      real(c_float), value :: dxi, dyi, dzi, dt, rho0
+      type(c_ptr), value :: dzci, p, u, v, w, rho
+      integer(c_int) :: rc
+    end function
+    function flutas_chkdiv_f32(nx, ny, nz, dxi, dyi, dzi, nh_d, nh_u, dzfi, u, v, w, divtot, divmax) &
+        bind(C, name="flutas_chkdiv_f32") result(rc)
+      import :: c_int, c_float, c_ptr
+      integer(c_int), value :: nx, ny, nz, nh_d, nh_u
+      real(c_float), value :: dxi, dyi, dzi
+      type(c_ptr), value :: dzfi, u, v, w
+      real(c_float) :: divtot, divmax
+      integer(c_int) :: rc
+    end function
+    function flutas_boundp_halo_f32(cbc, n, bc, nh_d, nh_p, dl, dzc, dzf, p) &
+        bind(C, name="flutas_boundp_halo_f32") result(rc)
+      import :: c_int, c_float, c_char, c_ptr
+      character(kind=c_char), dimension(6) :: cbc
+      integer(c_int), dimension(3) :: n
+      real(c_float), dimension(6) :: bc
+      integer(c_int), value :: nh_d, nh_p
+      real(c_float), dimension(3) :: dl
+      real(c_float), dimension(*) :: dzc, dzf
+      type(c_ptr), value :: p
+      integer(c_int) :: rc
+    end function
+    function flutas_bounduvw_f32(cbc, n, bc, nh_d, nh_u, isoutflow, dl, dzc, dzf, u, v, w) &
+        bind(C, name="flutas_bounduvw_f32") result(rc)
+      import :: c_int, c_float, c_char, c_ptr
+      character(kind=c_char), dimension(18) :: cbc
+      integer(c_int), dimension(3) :: n
+      real(c_float), dimension(18) :: bc
+      integer(c_int), value :: nh_d, nh_u
+      integer(c_int), dimension(6) :: isoutflow
+      real(c_float), dimension(3) :: dl
+      real(c_float), dimension(*) :: dzc, dzf
+      type(c_ptr), value :: u, v, w
+      integer(c_int) :: rc
+    end function
+    function flutas_updt_rhs_b_f32(nx, ny, nz, c_or_f, cbc, rhsbx, rhsby, rhsbz, p) &
+        bind(C, name="flutas_updt_rhs_b_f32") result(rc)
+      import :: c_int, c_char, c_ptr
+      integer(c_int), value :: nx, ny, nz
+      character(kind=c_char), dimension(3) :: c_or_f
+      character(kind=c_char), dimension(6) :: cbc
+      type(c_ptr), value :: rhsbx, rhsby, rhsbz, p
       integer(c_int) :: rc
     end function
   end interface

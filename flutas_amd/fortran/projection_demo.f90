@@ -19,9 +19,11 @@
 ! device buffers); "device" passes HIP device allocations.  SOLVER: "gpu"
 ! (default) calls solver_gpu, "cpu" the drop-in solver_cpu
 ! (src/solver_cpu.f90:20-31), as the reference's CPU builds do.
-! Built twice: with _CONSTANT_COEFFS_POISSON (projection_demo) and without
-! (projection_demo_vc: fillps without rho0, correc dividing by the face
-! density of rho = rho0).
+! Built three times: with _CONSTANT_COEFFS_POISSON (projection_demo),
+! without (projection_demo_vc: fillps without rho0, correc dividing by the
+! face density of rho = rho0), and with _CONSTANT_COEFFS_POISSON and
+! _SINGLE_PRECISION (projection_demo_sp: rp = FP32; the FP64 input is
+! rounded to FP32 and OUTPUT holds FP32 values).
 program projection_demo
   use, intrinsic :: iso_c_binding
   use flutas_capi
@@ -33,11 +35,14 @@ program projection_demo
   use mod_correc,     only: correc
   use mod_chkdiv,     only: chkdiv
   use mod_bound,      only: bounduvw, boundp, updt_rhs_b
+  use mod_initgrid,   only: initgrid
+  use flutas_kinds,   only: rp, rp_bytes
   implicit none
-  integer, parameter :: rp = c_double
   character(len=512) :: fin, fout, mode, solver
   integer(c_int32_t) :: hdr(4)
   real(rp) :: par(6)
+  real(c_double) :: par8(6)
+  real(c_double), allocatable :: f8(:,:,:)
   character(len=1) :: cbc(0:1,3), cbc6(6), cbcvel(0:1,3,3)
   real(rp) :: bc(0:1,3), bcvel(0:1,3,3)
   logical :: no_outflow(0:1,3)
@@ -53,7 +58,8 @@ program projection_demo
   solver = 'gpu'
   if (command_argument_count() >= 4) call get_command_argument(4, solver)
   open(newunit=u_in, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(u_in) hdr, par, cbc6
+  read(u_in) hdr, par8, cbc6
+  par = real(par8, rp)
   n = hdr(1:3)
   nh_u = hdr(4)
   nh_d = max(nh_u, 1)
@@ -72,12 +78,19 @@ program projection_demo
   allocate(v, w, mold=u)
   allocate(p(0:n(1)+1,0:n(2)+1,0:n(3)+1))
   allocate(rho, f, mold=p)
-  read(u_in) u, v, w
+  allocate(f8(1-nh_u:n(1)+nh_u,1-nh_u:n(2)+nh_u,1-nh_u:n(3)+nh_u))
+  read(u_in) f8
+  u = real(f8, rp)
+  read(u_in) f8
+  v = real(f8, rp)
+  read(u_in) f8
+  w = real(f8, rp)
+  deallocate(f8)
   close(u_in)
   p = 0._rp
   rho = par(6)
-  bu = 8_c_size_t*size(u, kind=c_size_t)
-  bp = 8_c_size_t*size(p, kind=c_size_t)
+  bu = rp_bytes*size(u, kind=c_size_t)
+  bp = rp_bytes*size(p, kind=c_size_t)
   !
   if (trim(mode) == 'device') then
     call flutas_check(flutas_device_alloc(bu, cu), 'alloc u')
@@ -120,8 +133,7 @@ contains
     integer :: u_div
     allocate(dzc(1-nh_d:n(3)+nh_d), dzf(1-nh_d:n(3)+nh_d), zc(1-nh_d:n(3)+nh_d), zf(1-nh_d:n(3)+nh_d))
     allocate(dzci, dzfi, mold=dzc)
-    call flutas_check(flutas_initgrid(int(n(3), c_int), par(4), par(3), int(nh_d, c_int), dzc, dzf, zc, zf), &
-                      'initgrid')
+    call initgrid('zer', n(3), par(4), par(3), nh_d, dzc, dzf, zc, zf)
     dzci(:) = 1._rp/dzc(:)
     dzfi(:) = 1._rp/dzf(:)
     dli = real(n, rp)/par(1:3)
@@ -158,7 +170,7 @@ contains
     integer(c_int) :: isdev
     call flutas_check(flutas_pointer_is_device(c_loc(src), isdev), 'pointer query')
     if (isdev == 1) then
-      call flutas_check(flutas_memcpy_d2h(c_loc(dst), c_loc(src), 8_c_size_t*size(src, kind=c_size_t)), &
+      call flutas_check(flutas_memcpy_d2h(c_loc(dst), c_loc(src), rp_bytes*size(src, kind=c_size_t)), &
                         'snapshot')
     else
       dst = src

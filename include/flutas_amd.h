@@ -135,6 +135,52 @@ int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
                           int reps, int max_stages, double *ms, int *nstages);
 const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 
+/* ---- FP32 fields: FluTAS built with _SINGLE_PRECISION (rp = FP32,
+ *      src/types.f90:6-12) ------------------------------------------------
+ * The same entry points for FP32 fields and FP32 scalars, computing in
+ * FP32 as the reference's single-precision build does (chkdiv sums in
+ * FP64).  The solve runs the FP32 chain (FP32 transforms, FP64 z
+ * elimination) once on a plan of either precision, in place on the
+ * interior of p(0:nx+1,0:ny+1,0:nz+1); a non-finite right-hand side
+ * returns FLUTAS_ENONFINITE on the four-pass pipeline. */
+int flutas_psolve_solve_f32(flutas_psolve *h, float *p);
+int flutas_fft_guru_f32(int kind, int n, long long stride, int howmany1,
+                        long long dist1, int howmany2, long long dist2,
+                        float *data);
+int flutas_fillps_f32(int nx, int ny, int nz, int nh_d, int nh_u, float dxi,
+                      float dyi, float dzi, const float *dzfi, float dti,
+                      float rho0, const float *u, const float *v,
+                      const float *w, float *p);
+int flutas_pres_tw_src_f32(int nx, int ny, int nz, float dxi, float dyi,
+                           float dzi, int nh_d, int nh_u, const float *dzci,
+                           float rho0i, float f_t12, float f_t12_o,
+                           const float *p, const float *pold, const float *rho,
+                           float *u, float *v, float *w, int const_coeff);
+int flutas_pres_sp_src_f32(int nx, int ny, int nz, float f_t12, float dxi,
+                           float dyi, float dzi, int nh_d, int nh_u,
+                           const float *dzci, float rho0i, const float *pold,
+                           float *u, float *v, float *w);
+int flutas_correc_f32(int nx, int ny, int nz, int nh_d, int nh_u, float dxi,
+                      float dyi, float dzi, const float *dzci, float dt,
+                      float rho0, const float *p, float *u, float *v,
+                      float *w, const float *rho, int const_coeff);
+int flutas_chkdiv_f32(int nx, int ny, int nz, float dxi, float dyi, float dzi,
+                      int nh_d, int nh_u, const float *dzfi, const float *u,
+                      const float *v, const float *w, float *divtot,
+                      float *divmax);
+int flutas_boundp_f32(const char cbc[6], const int n[3], const float bc[6],
+                      const float dl[3], float dzc_0, float dzc_n, float *p);
+int flutas_boundp_halo_f32(const char cbc[6], const int n[3], const float bc[6],
+                           int nh_d, int nh_p, const float dl[3],
+                           const float *dzc, const float *dzf, float *p);
+int flutas_bounduvw_f32(const char cbc[18], const int n[3], const float bc[18],
+                        int nh_d, int nh_u, const int isoutflow[6],
+                        const float dl[3], const float *dzc, const float *dzf,
+                        float *u, float *v, float *w);
+int flutas_updt_rhs_b_f32(int nx, int ny, int nz, const char c_or_f[3],
+                          const char cbc[6], const float *rhsbx,
+                          const float *rhsby, const float *rhsbz, float *p);
+
 /* ---- graphs of library calls (no counterpart in the reference, whose
  *      steps launch kernel by kernel) --------------------------------------
  * Between flutas_graph_begin and flutas_graph_end the asynchronous library

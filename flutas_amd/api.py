@@ -228,6 +228,15 @@ class PoissonPlan:
         return [(lib.flutas_psolve_stage_name(_VP(self._h), s).decode(), ms[s])
                 for s in range(ns.value)]
 
+    def time_stage(self, f, p, stage, reps=8):
+        """Average duration (ms) of one launch of stage 0..3 of the FP64
+        four-pass pipeline, launched `reps` times back to back after a
+        solve f -> p (flutas_psolve_time_stage)."""
+        ms = C.c_double()
+        _check(lib.flutas_psolve_time_stage(_VP(self._h), f.c, p.c, C.c_int(stage),
+                                            C.c_int(reps), C.byref(ms)))
+        return ms.value
+
     def rhsb_device(self):
         rx, ry, rz = _D(), _D(), _D()
         _check(lib.flutas_psolve_rhsb_device(_VP(self._h), C.byref(rx),

@@ -827,6 +827,56 @@ int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
   return FLUTAS_OK;
 }
 
+// One stage of the four-pass pipeline launched `reps` times back to back
+// between two events on the library stream (after one whole solve f -> p
+// has filled the work field and the checkpoints): the kernel's own average
+// duration, free of the gaps that events between stages add.  P2 and P3
+// run in place on the work field, so their repeats transform their own
+// output again (no overflow for reps <= 16; the data of the work field are
+// not used afterwards).
+int flutas_psolve_time_stage(flutas_psolve *h, const double *f, double *p, int stage,
+                             int reps, double *ms) {
+  if (!h || !f || !p || !ms || reps < 1 || reps > 16 || stage < 0 || stage > 3)
+    return fail(FLUTAS_EINVAL, "bad stage timing arguments");
+  if (!h->fused || h->precision != FLUTAS_FP64)
+    return fail(FLUTAS_EUNSUPPORTED, "stage timing covers the four-pass FP64 pipeline");
+  hipStream_t s = current_stream();
+  FL_TRY(solve_impl(h, f, p));
+  const Geo q = halo_geo(h);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const int per = (h->cbc[4] == 'P');
+  const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
+  const WStride ws = fused_layout_fp64(nx, ny, nz);
+  hipEvent_t e0, e1;
+  FL_HIP(hipEventCreate(&e0));
+  FL_HIP(hipEventCreate(&e1));
+  int rc = FLUTAS_OK;
+  if (hipEventRecord(e0, s) != hipSuccess) rc = fail(FLUTAS_EDEVICE, "hipEventRecord");
+  for (int r = 0; r < reps && rc == FLUTAS_OK; ++r) {
+    if (stage == 0)
+      rc = launch_x_fwd<double, double>(h->tx, h->kinds[0], f + q.interior, q.hx, q.hxy, h->W64,
+                                        ws, ny, nz, nullptr, nullptr, 0, s);
+    else if (stage == 1)
+      rc = launch_yz_fwd<double>(h->ty, h->kinds[2], h->W64, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
+                                 h->d_lamy, h->d_a, h->d_b, h->d_c, pin, per, s);
+    else if (stage == 2)
+      rc = launch_zy_bwd<double>(h->ty, h->kinds[3], h->W64, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
+                                 h->d_lamy, h->d_a, h->d_b, h->d_c, per, s);
+    else
+      rc = launch_x_bwd<double, double>(h->tx, h->kinds[1], h->W64, ws, p + q.interior, q.hx,
+                                        q.hxy, ny, nz, h->normfft, 0, s);
+  }
+  float t = 0.f;
+  if (rc == FLUTAS_OK && hipEventRecord(e1, s) != hipSuccess) rc = fail(FLUTAS_EDEVICE, "hipEventRecord");
+  if (rc == FLUTAS_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(FLUTAS_EDEVICE, "hipEventSynchronize");
+  if (rc == FLUTAS_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = fail(FLUTAS_EDEVICE, "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *ms = (double)t / reps;
+  return FLUTAS_OK;
+}
+
 const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage) {
   if (!h || stage < 0 || stage >= (int)h->stage_names.size()) return "";
   return h->stage_names[stage].c_str();

@@ -134,6 +134,13 @@ int flutas_psolve_residual(flutas_psolve *h, const double *f, const double *p,
 int flutas_psolve_profile(flutas_psolve *h, const double *f, double *p,
                           int reps, int max_stages, double *ms, int *nstages);
 const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
+/* One stage of the four-pass FP64 pipeline (0 x forward, 1 y forward and
+ * z elimination, 2 z substitution and y backward, 3 x backward) launched
+ * `reps` (<= 16) times back to back after one solve f -> p: *ms = the
+ * average duration of one launch, from two HIP events on the library
+ * stream.  The work field is left transformed; p holds the solution. */
+int flutas_psolve_time_stage(flutas_psolve *h, const double *f, double *p, int stage,
+                             int reps, double *ms);
 
 /* ---- FP32 fields: FluTAS built with _SINGLE_PRECISION (rp = FP32,
  *      src/types.f90:6-12) ------------------------------------------------

@@ -237,6 +237,28 @@ class PoissonPlan:
                                             C.c_int(reps), C.byref(ms)))
         return ms.value
 
+    def set_zchunks(self, n):
+        """Chunks per x-slab of the z-chunked slab passes (1 = whole slabs,
+        0 = the plan's default); returns the count in use
+        (flutas_psolve_set_zchunks)."""
+        _check(lib.flutas_psolve_set_zchunks(_VP(self._h), C.c_int(n)))
+        return self.zchunks
+
+    @property
+    def zchunks(self):
+        n = C.c_int()
+        _check(lib.flutas_psolve_zchunks(_VP(self._h), C.byref(n)))
+        return n.value
+
+    def tune(self, f, p, reps=5):
+        """Times `reps` solves f -> p at 1, 2, 4, 8 z chunks, keeps the
+        fastest (flutas_psolve_tune); returns (chosen, {count: ms})."""
+        chosen = C.c_int()
+        ms = (C.c_double * 4)()
+        _check(lib.flutas_psolve_tune(_VP(self._h), f.c, p.c, C.c_int(reps),
+                                      C.byref(chosen), ms))
+        return chosen.value, {n: ms[v] for v, n in enumerate((1, 2, 4, 8)) if ms[v] >= 0}
+
     def rhsb_device(self):
         rx, ry, rz = _D(), _D(), _D()
         _check(lib.flutas_psolve_rhsb_device(_VP(self._h), C.byref(rx),

@@ -2,6 +2,7 @@
 // pressure solve; the device bodies and the pipeline description are in
 // fused_body.hpp.
 #include "fused_body.hpp"
+#include "fused_chunk.hpp"
 
 namespace flutas {
 
@@ -78,6 +79,149 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
   __shared__ __align__(16) T stw[Tw<T, NC>::total];
   zy_bwd_slab<T, NC, GP, NT, PER>(Wt, ws, ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, tw, twh,
                                   twq, sm, stw, blockIdx.x);
+}
+
+template <typename T, int NC, int GP, int NT, bool PER>
+__global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
+    k_yz_fwd_zc(T *__restrict__ Wt, WStride ws, ZChunks zc, int nx, int nz, int msys, int kind,
+                const double *__restrict__ lamx, const double *__restrict__ lamy,
+                const double *__restrict__ a, const double *__restrict__ b,
+                const double *__restrict__ c, int pin, const T *__restrict__ tw,
+                const T *__restrict__ twh, const T *__restrict__ twq) {
+  __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  int i, ch;
+  zc_slab_of(nx, zc.nchunk, &i, &ch);
+  yz_fwd_chunk<T, NC, GP, NT, PER>(Wt, ws, zc, nz, msys, kind, lamx, lamy, a, b, c, pin, tw, twh,
+                                   twq, sm, stw, i, ch);
+}
+
+template <typename T, int NC, int GP, int NT, bool PER>
+__global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
+    k_zy_bwd_zc(T *__restrict__ Wt, WStride ws, ZChunks zc, int nx, int nz, int msys, int kind,
+                const double *__restrict__ lamx, const double *__restrict__ lamy,
+                const double *__restrict__ a, const double *__restrict__ b,
+                const double *__restrict__ c, int pin, const T *__restrict__ tw,
+                const T *__restrict__ twh, const T *__restrict__ twq) {
+  __shared__ __align__(16) T sm[2 * GP * line_stride<T, 2 * NC>()];
+  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  int i, ch;
+  zc_slab_of(nx, zc.nchunk, &i, &ch);
+  zy_bwd_chunk<T, NC, GP, NT, PER>(Wt, ws, zc, nz, msys, kind, lamx, lamy, a, b, c, pin, tw, twh,
+                                   twq, sm, stw, i, ch);
+}
+
+// Tables of the z-chunked passes, one thread per column (i, j), with the
+// operations of yz_fwd_chunk / zy_bwd_chunk (so d is bit for bit the
+// passes' and the reference's): walking up, d, m (periodic: q2) before
+// every plane group and the chunk sums M, SW, N; for periodic z then
+// walking down group by group (factors recomputed from the stored values),
+// the second solution p2 at every chunk start and the closing row's
+// denominator (b(m) + lam) + c(m) p2(0) + a(m) p2(m-1).
+constexpr int kZcMaxGroup = 16;
+__global__ void __launch_bounds__(256)
+    k_zchunk_tables(double *__restrict__ ckd, double *__restrict__ ckm, double *__restrict__ ckq,
+                    double *__restrict__ tab, double *__restrict__ den, int nx, int ny, int nz,
+                    int msys, int gp, int cpl, int nchunk, int per, int pin,
+                    const double *__restrict__ lamx, const double *__restrict__ lamy,
+                    const double *__restrict__ a, const double *__restrict__ b,
+                    const double *__restrict__ c) {
+#pragma clang fp contract(off)
+  const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= (long long)nx * ny) return;
+  const int i = (int)(col / ny), j = (int)(col - (long long)i * ny);
+  const double lam = lamx[i] + lamy[j];
+  const int ngrp = (nz + gp - 1) / gp;
+  const int TS = zc_tab_slots(per != 0);
+  double dp = 0.0, mp = 1.0, qp = 0.0, w = 1.0, sw = 0.0, ml = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    const int ch = min(k / cpl, nchunk - 1);
+    if (k == ch * cpl) {
+      mp = 1.0;
+      w = 1.0;
+      sw = 0.0;
+      ml = 0.0;
+    }
+    if (k % gp == 0) {
+      const long long o = ((long long)i * ngrp + k / gp) * ny + j;
+      ckd[o] = dp;
+      ckm[o] = mp;
+      if (per) ckq[o] = qp;
+    }
+    if (k < msys) {
+      const double r2 = per ? ((k == msys - 1) ? -c[msys - 1] : ((k == 0) ? -a[0] : 0.0)) : 0.0;
+      double dl = 0.0;
+      if (k == 0) {
+        const double z = 1.0 / (b[0] + lam);
+        dp = c[0] * z;
+        mp = 0.0;
+        qp = r2 * z;
+        dl = dp;
+      } else if (k < msys - 1) {
+        const double z = 1.0 / ((b[k] + lam) - a[k] * dp);
+        dp = c[k] * z;
+        mp = (-a[k] * mp) * z;
+        qp = (r2 - a[k] * qp) * z;
+        dl = dp;
+      } else {
+        const double z = (b[k] + lam) - a[k] * dp;
+        mp = (z != 0.0 && (per || !(pin && lam == 0.0))) ? (-a[k] * mp) / z : 0.0;
+        qp = (z != 0.0) ? (r2 - a[k] * qp) / z : 0.0;
+      }
+      sw = sw + w * mp;
+      w = -(dl * w);
+      ml = mp;
+    }
+    const int ke = (ch == nchunk - 1) ? nz : (ch + 1) * cpl;
+    if (k == ke - 1) {
+      double *t = tab + ((long long)i * nchunk + ch) * TS * ny + j;
+      t[0] = ml;
+      t[ny] = sw;
+      t[2 * ny] = w;
+    }
+  }
+  if (!per) return;
+  const int m = msys;  // nz - 1
+  double p2n = 0.0, p20 = 0.0, p2m1 = 0.0;
+  for (int g = ngrp - 1; g >= 0; --g) {
+    const int k0 = g * gp;
+    const long long o = ((long long)i * ngrp + g) * ny + j;
+    double dd[kZcMaxGroup], qq[kZcMaxGroup];
+    double d2 = ckd[o], q2 = ckq[o];
+#pragma unroll
+    for (int l = 0; l < kZcMaxGroup; ++l) {
+      const int k = k0 + l;
+      if (l < gp && k < m) {
+        const double r2 = (k == m - 1) ? -c[m - 1] : ((k == 0) ? -a[0] : 0.0);
+        if (k == 0) {
+          const double z = 1.0 / (b[0] + lam);
+          d2 = c[0] * z;
+          q2 = r2 * z;
+        } else if (k < m - 1) {
+          const double z = 1.0 / ((b[k] + lam) - a[k] * d2);
+          d2 = c[k] * z;
+          q2 = (r2 - a[k] * q2) * z;
+        } else {
+          const double z = (b[k] + lam) - a[k] * d2;
+          q2 = (z != 0.0) ? (r2 - a[k] * q2) / z : 0.0;
+        }
+      }
+      dd[l] = d2;
+      qq[l] = q2;
+    }
+#pragma unroll
+    for (int l = kZcMaxGroup - 1; l >= 0; --l) {
+      const int k = k0 + l;
+      if (l >= gp || k >= m) continue;
+      const double p2 = (k == m - 1) ? qq[l] : qq[l] - dd[l] * p2n;
+      p2n = p2;
+      if (k == m - 1) p2m1 = p2;
+      if (k == 0) p20 = p2;
+      if (k > 0 && k % cpl == 0 && k / cpl < nchunk)
+        tab[(((long long)i * nchunk + k / cpl - 1) * TS + 3) * ny + j] = p2;
+    }
+  }
+  den[(long long)i * ny + j] = ((b[m] + lam) + c[m] * p20) + a[m] * p2m1;
 }
 
 // ---------------------------------------------------------------- host ---
@@ -275,6 +419,81 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
   return FLUTAS_OK;
 }
 
+size_t zchunk_doubles(int nx, int ny, int ngrp, int nchunk, bool per) {
+  const size_t col = (size_t)nx * ny;
+  return col * (3 * (size_t)ngrp + (size_t)nchunk * zc_tab_slots(per) + 1 +
+                (size_t)nchunk * zc_side_slots(per));
+}
+
+int build_zchunk_tables(double *mem, ZChunks *zc, int nx, int ny, int nz, int msys, int gp,
+                        int cpl, int nchunk, bool per, int pin, const double *lamx,
+                        const double *lamy, const double *a, const double *b, const double *c,
+                        hipStream_t s) {
+  if (gp < 1 || gp > kZcMaxGroup || cpl % gp != 0 || nchunk < 2 || nchunk > kMaxZChunks ||
+      (nchunk - 1) * cpl > nz - 2)
+    return fail(FLUTAS_EINVAL, "z chunks: bad chunk shape");
+  const int ngrp = (nz + gp - 1) / gp;
+  const size_t col = (size_t)nx * ny;
+  double *ckd = mem, *ckm = ckd + col * ngrp, *ckq = ckm + col * ngrp;
+  double *tab = ckq + col * ngrp;
+  double *den = tab + col * nchunk * zc_tab_slots(per);
+  double *side = den + col;
+  const int threads = 256;
+  const long long blocks = ((long long)col + threads - 1) / threads;
+  hipLaunchKernelGGL(k_zchunk_tables, dim3((unsigned)blocks), dim3(threads), 0, s, ckd, ckm, ckq,
+                     tab, den, nx, ny, nz, msys, gp, cpl, nchunk, per ? 1 : 0, pin, lamx, lamy,
+                     a, b, c);
+  FL_HIP(hipGetLastError());
+  *zc = ZChunks{nchunk, cpl, ngrp, ckd, ckm, ckq, tab, den, side};
+  return FLUTAS_OK;
+}
+
+template <typename T>
+int launch_yz_fwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZChunks &zc,
+                     int nx, int nz, int msys, const double *lamx, const double *lamy,
+                     const double *a, const double *b, const double *c, int pin, int periodic,
+                     hipStream_t s) {
+  const LinePlan<T> pl = plan_of<T>(ty);
+  const dim3 grid(nx * zc.nchunk);
+  FL_DISPATCH(ty.n / 2, {
+    constexpr int GP = group_planes<T, NC>();
+    constexpr int NTP = slab_threads<T, NC, true>();
+    constexpr int NTN = slab_threads<T, NC, false>();
+    if (zc.ngrp != (nz + GP - 1) / GP) return fail(FLUTAS_EINVAL, "z chunks: plane groups");
+    if (periodic)
+      hipLaunchKernelGGL((k_yz_fwd_zc<T, NC, GP, NTP, true>), grid, dim3(NTP), 0, s, Wt, ws, zc,
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+    else
+      hipLaunchKernelGGL((k_yz_fwd_zc<T, NC, GP, NTN, false>), grid, dim3(NTN), 0, s, Wt, ws, zc,
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+  });
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
+template <typename T>
+int launch_zy_bwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZChunks &zc,
+                     int nx, int nz, int msys, const double *lamx, const double *lamy,
+                     const double *a, const double *b, const double *c, int pin, int periodic,
+                     hipStream_t s) {
+  const LinePlan<T> pl = plan_of<T>(ty);
+  const dim3 grid(nx * zc.nchunk);
+  FL_DISPATCH(ty.n / 2, {
+    constexpr int GP = group_planes<T, NC>();
+    constexpr int NTP = slab_threads<T, NC, true>();
+    constexpr int NTN = slab_threads<T, NC, false>();
+    if (zc.ngrp != (nz + GP - 1) / GP) return fail(FLUTAS_EINVAL, "z chunks: plane groups");
+    if (periodic)
+      hipLaunchKernelGGL((k_zy_bwd_zc<T, NC, GP, NTP, true>), grid, dim3(NTP), 0, s, Wt, ws, zc,
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+    else
+      hipLaunchKernelGGL((k_zy_bwd_zc<T, NC, GP, NTN, false>), grid, dim3(NTN), 0, s, Wt, ws, zc,
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+  });
+  FL_HIP(hipGetLastError());
+  return FLUTAS_OK;
+}
+
 template int launch_x_fwd<double, double>(const LineTables &, int, const double *, long long, long long, double *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_fwd<float, double>(const LineTables &, int, const double *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
 template int launch_x_fwd<float, float>(const LineTables &, int, const float *, long long, long long, float *, WStride, int, int, double *, int *, int, hipStream_t);
@@ -285,5 +504,10 @@ template int launch_yz_fwd<double>(const LineTables &, int, double *, WStride, d
 template int launch_yz_fwd<float>(const LineTables &, int, float *, WStride, double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
 template int launch_zy_bwd<double>(const LineTables &, int, double *, WStride, const double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
 template int launch_zy_bwd<float>(const LineTables &, int, float *, WStride, const double *, int, int, int, const double *, const double *, const double *, const double *, const double *, int, hipStream_t);
+
+template int launch_yz_fwd_zc<double>(const LineTables &, int, double *, WStride, const ZChunks &, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_yz_fwd_zc<float>(const LineTables &, int, float *, WStride, const ZChunks &, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_zy_bwd_zc<double>(const LineTables &, int, double *, WStride, const ZChunks &, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
+template int launch_zy_bwd_zc<float>(const LineTables &, int, float *, WStride, const ZChunks &, int, int, int, const double *, const double *, const double *, const double *, const double *, int, int, hipStream_t);
 
 }  // namespace flutas

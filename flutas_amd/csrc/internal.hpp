@@ -141,6 +141,42 @@ __host__ __device__ inline WStride fused_layout_fp64(int nx, int ny, int nz) {
 }
 bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
+// Tables and side arrays of the z-chunked slab passes (fused_chunk.hpp):
+// the planes of an x-slab are cut into nchunk chunks of cpl planes (a
+// multiple of the plane group; the last chunk runs to nz).  Index forms:
+//   ckd, ckm, ckq  [(i * ngrp + g) * ny + j]   value at the plane before
+//                  plane group g: d, the chunk's carry multiplier m, the
+//                  periodic q2
+//   tab            [((i * nchunk + c) * zc_tab_slots + s) * ny + j]
+//                  slots M, SW, N (periodic: p2 of the plane after chunk c)
+//   den            [i * ny + j]  periodic: the closing row's denominator
+//   side           [((i * nchunk + c) * zc_side_slots + s) * ny + j]
+//                  slots E, S (periodic: y of the closing plane, last chunk)
+constexpr int kMaxZChunks = 8;
+struct ZChunks {
+  int nchunk, cpl, ngrp;
+  const double *ckd, *ckm, *ckq, *tab, *den;
+  double *side;
+};
+__host__ __device__ constexpr int zc_tab_slots(bool per) { return per ? 4 : 3; }
+__host__ __device__ constexpr int zc_side_slots(bool per) { return per ? 3 : 2; }
+// Tables of a plan's chunking (one thread per column; mem holds
+// 3 ngrp + nchunk tab slots + 1 + nchunk side slots values per column).
+size_t zchunk_doubles(int nx, int ny, int ngrp, int nchunk, bool per);
+int build_zchunk_tables(double *mem, ZChunks *zc, int nx, int ny, int nz, int msys, int gp,
+                        int cpl, int nchunk, bool per, int pin, const double *lamx,
+                        const double *lamy, const double *a, const double *b, const double *c,
+                        hipStream_t s);
+template <typename T>
+int launch_yz_fwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZChunks &zc,
+                     int nx, int nz, int msys, const double *lamx, const double *lamy,
+                     const double *a, const double *b, const double *c, int pin, int periodic,
+                     hipStream_t s);
+template <typename T>
+int launch_zy_bwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZChunks &zc,
+                     int nx, int nz, int msys, const double *lamx, const double *lamy,
+                     const double *a, const double *b, const double *c, int pin, int periodic,
+                     hipStream_t s);
 // Uploads the per-pass twiddle tables of the compiled plan of tab.n.
 int build_fused_twiddles(LineTables &tab);
 // absmax_part: one max|in| partial per workgroup (or null); nf_flag: set

@@ -15,6 +15,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
@@ -163,6 +164,12 @@ struct flutas_psolve {
   size_t nres = 0, nx1 = 0;
   bool fused = false;      // four-pass pipeline (fused.hip) in use
   double *CK = nullptr;    // z elimination checkpoints of the fused pipeline
+  // z-chunked slab passes (fused_chunk.hpp): chunks per x-slab (1: whole
+  // slabs), their tables, and the element size of the chain they serve
+  int zchunks = 1;
+  ZChunks zc{};
+  double *zc_mem = nullptr;
+  int zc_elem = 0;
   int last_iters = 0;
   double last_relres = -1.0;
   // NaN/Inf detection of the FP64 solve: P1 (or a flag pass on the line
@@ -193,7 +200,7 @@ void release(flutas_psolve *h) {
                   (void *)h->d_b, (void *)h->d_c, (void *)h->d_rhsbx,
                   (void *)h->d_rhsby, (void *)h->d_rhsbz, (void *)h->W64,
                   (void *)h->W32, (void *)h->D, (void *)h->P2, (void *)h->F,
-                  (void *)h->R32, (void *)h->part, (void *)h->CK})
+                  (void *)h->R32, (void *)h->part, (void *)h->CK, (void *)h->zc_mem})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
@@ -280,6 +287,82 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
                           h->nx, h->ny, h->nzs, pin, s);
 }
 
+// The slab passes P2/P3 of the fused pipeline: z-chunked when the plan
+// has chunk tables for this chain's element size, whole slabs otherwise.
+template <typename T>
+int slab_fwd(flutas_psolve *h, T *W, WStride ws, int pin, int per, hipStream_t s) {
+  if (h->zchunks > 1 && h->zc_elem == (int)sizeof(T))
+    return launch_yz_fwd_zc<T>(h->ty, h->kinds[2], W, ws, h->zc, h->nx, h->nz,
+                               per ? h->nz - 1 : h->nzs, h->d_lamx, h->d_lamy, h->d_a, h->d_b,
+                               h->d_c, pin, per, s);
+  return launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, h->nx, h->nz, h->nzs, h->d_lamx,
+                          h->d_lamy, h->d_a, h->d_b, h->d_c, pin, per, s);
+}
+template <typename T>
+int slab_bwd(flutas_psolve *h, T *W, WStride ws, int pin, int per, hipStream_t s) {
+  if (h->zchunks > 1 && h->zc_elem == (int)sizeof(T))
+    return launch_zy_bwd_zc<T>(h->ty, h->kinds[3], W, ws, h->zc, h->nx, h->nz,
+                               per ? h->nz - 1 : h->nzs, h->d_lamx, h->d_lamy, h->d_a, h->d_b,
+                               h->d_c, pin, per, s);
+  return launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, h->nx, h->nz, h->nzs, h->d_lamx,
+                          h->d_lamy, h->d_a, h->d_b, h->d_c, per, s);
+}
+
+// Chunks per x-slab for the z-chunked slab passes: n = 1 turns them off.
+// The planes are cut at multiples of the plane group into n chunks (fewer
+// when nz is short; the last chunk runs to nz and holds two planes or
+// more), and the tables are computed for the plan's chain.
+int set_zchunks(flutas_psolve *h, int n) {
+  if (n < 1 || n > kMaxZChunks) return fail(FLUTAS_EINVAL, "z chunks must be 1..8");
+  if (h->zc_mem) {
+    FL_HIP(hipDeviceSynchronize());
+    (void)hipFree(h->zc_mem);
+    h->zc_mem = nullptr;
+  }
+  h->zchunks = 1;
+  h->zc = ZChunks{};
+  h->zc_elem = 0;
+  if (n == 1 || !h->fused) return FLUTAS_OK;
+  const int elem = (h->precision == FLUTAS_FP64) ? 8 : 4;
+  const int gp = fused_group_planes(h->ny, elem);
+  const int nz = h->nz;
+  const int per = (h->cbc[4] == 'P');
+  const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
+  // planes per chunk: a multiple of gp near nz / n
+  const int cpl = gp * std::max(1, (nz / n + gp / 2) / gp);
+  int nch = std::min(n, (nz + cpl - 1) / cpl);
+  while (nch > 1 && (nch - 1) * cpl > nz - 2) --nch;
+  if (nch < 2) return FLUTAS_OK;
+  const int ngrp = (nz + gp - 1) / gp;
+  const size_t nd = zchunk_doubles(h->nx, h->ny, ngrp, nch, per != 0);
+  FL_HIP(hipMalloc((void **)&h->zc_mem, nd * sizeof(double)));
+  const int rc = build_zchunk_tables(h->zc_mem, &h->zc, h->nx, h->ny, nz, per ? nz - 1 : h->nzs,
+                                     gp, cpl, nch, per != 0, pin, h->d_lamx, h->d_lamy, h->d_a,
+                                     h->d_b, h->d_c, nullptr);
+  if (rc == FLUTAS_OK && hipDeviceSynchronize() != hipSuccess)
+    return fail(FLUTAS_EDEVICE, "z chunk tables");
+  if (rc) {
+    (void)hipFree(h->zc_mem);
+    h->zc_mem = nullptr;
+    return rc;
+  }
+  h->zchunks = nch;
+  h->zc_elem = elem;
+  return FLUTAS_OK;
+}
+
+// Default chunking: 512 slab workgroups or more (two per CU, as many as
+// the LDS of the 256- and 512-point passes holds at once) when the plan
+// has fewer x-slabs, with chunks of 32 planes or more.  Measured FP64 at
+// 256x256x512: 1 chunk 0.78 ms, 2 chunks 0.56 ms, 4 chunks 0.58 ms; at
+// 128^3: 0.115, 0.082, 0.065 (4), 0.072 (8) ms; at 512^3 whole slabs are
+// fastest.
+int default_zchunks(const flutas_psolve *h) {
+  int n = 1;
+  while (n < kMaxZChunks && h->nx * n < 512 && h->nz / (2 * n) >= 32) n *= 2;
+  return n;
+}
+
 // One spectral solve in storage/arithmetic T: the right-hand side `in`
 // (halo-1 layout at its interior origin when in_halo, else dense nx*ny*nz)
 // -> out (interior origin of a halo-1 FP64 field): out = S(in), or
@@ -301,11 +384,9 @@ int spectral(flutas_psolve *h, const Tin *in, bool in_halo, Tout *out,
     FL_TRY((launch_x_fwd<T, Tin>(h->tx, h->kinds[0], in, is1, is2, W, ws, ny, nz, absmax_part,
                                  nf_flag, h->seq, s)));
     FL_TRY(mark(h, "P1_x_fwd", s));
-    FL_TRY(launch_yz_fwd<T>(h->ty, h->kinds[2], W, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
-                            h->d_lamy, h->d_a, h->d_b, h->d_c, pin, per, s));
+    FL_TRY(slab_fwd<T>(h, W, ws, pin, per, s));
     FL_TRY(mark(h, "P2_yz_fwd", s));
-    FL_TRY(launch_zy_bwd<T>(h->ty, h->kinds[3], W, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
-                            h->d_lamy, h->d_a, h->d_b, h->d_c, per, s));
+    FL_TRY(slab_bwd<T>(h, W, ws, pin, per, s));
     FL_TRY(mark(h, "P3_zy_bwd", s));
     FL_TRY((launch_x_bwd<T, Tout>(h->tx, h->kinds[1], W, ws, out, q.hx, q.hxy, ny, nz,
                                   (T)h->normfft, accumulate, s)));
@@ -623,6 +704,8 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
     h->nx1 = (h->fused && precision == FLUTAS_FP32_REFINE) ? (size_t)x_fwd_blocks(h->tx, 4, h->ny, h->nz) : 0;
     if (e == hipSuccess) e = hipMalloc((void **)&h->part, (h->nres + h->nx1 + 2) * sizeof(double));
     if (e != hipSuccess) { rc = fail(FLUTAS_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e)); break; }
+    const char *zenv = std::getenv("FLUTAS_ZCHUNKS");
+    if ((rc = set_zchunks(h, zenv ? std::atoi(zenv) : default_zchunks(h)))) break;
   } while (0);
   if (rc) {
     release(h);
@@ -857,11 +940,9 @@ int flutas_psolve_time_stage(flutas_psolve *h, const double *f, double *p, int s
       rc = launch_x_fwd<double, double>(h->tx, h->kinds[0], f + q.interior, q.hx, q.hxy, h->W64,
                                         ws, ny, nz, nullptr, nullptr, 0, s);
     else if (stage == 1)
-      rc = launch_yz_fwd<double>(h->ty, h->kinds[2], h->W64, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
-                                 h->d_lamy, h->d_a, h->d_b, h->d_c, pin, per, s);
+      rc = slab_fwd<double>(h, h->W64, ws, pin, per, s);
     else if (stage == 2)
-      rc = launch_zy_bwd<double>(h->ty, h->kinds[3], h->W64, ws, h->CK, nx, nz, h->nzs, h->d_lamx,
-                                 h->d_lamy, h->d_a, h->d_b, h->d_c, per, s);
+      rc = slab_bwd<double>(h, h->W64, ws, pin, per, s);
     else
       rc = launch_x_bwd<double, double>(h->tx, h->kinds[1], h->W64, ws, p + q.interior, q.hx,
                                         q.hxy, ny, nz, h->normfft, 0, s);
@@ -950,6 +1031,51 @@ int flutas_event_elapsed_ms(void *start, void *end, float *ms) {
 int flutas_event_destroy(void *ev) {
   if (ev) FL_HIP(hipEventDestroy((hipEvent_t)ev));
   return FLUTAS_OK;
+}
+
+int flutas_psolve_set_zchunks(flutas_psolve *h, int n) {
+  if (!h) return fail(FLUTAS_EINVAL, "null plan");
+  return set_zchunks(h, n == 0 ? default_zchunks(h) : n);
+}
+
+int flutas_psolve_zchunks(const flutas_psolve *h, int *n) {
+  if (!h || !n) return fail(FLUTAS_EINVAL, "null pointer");
+  *n = h->zchunks;
+  return FLUTAS_OK;
+}
+
+int flutas_psolve_tune(flutas_psolve *h, const double *f, double *p, int reps, int *chosen,
+                       double *ms) {
+  if (!h || !f || !p || !chosen || reps < 1 || reps > 100)
+    return fail(FLUTAS_EINVAL, "bad tuning arguments");
+  hipStream_t s = current_stream();
+  const int counts[4] = {1, 2, 4, 8};
+  int best = 1;
+  double tbest = 1e300;
+  // the first timed count should not pay for the GPU's clock ramp
+  for (int r = 0; r < 3; ++r) FL_TRY(solve_impl(h, f, p));
+  for (int v = 0; v < 4; ++v) {
+    if (ms) ms[v] = -1.0;
+    FL_TRY(set_zchunks(h, counts[v]));
+    if (h->zchunks != counts[v]) continue;
+    FL_TRY(solve_impl(h, f, p));  // warm-up
+    FL_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) FL_TRY(solve_impl(h, f, p));
+    FL_HIP(hipStreamSynchronize(s));
+    const double t = std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - t0).count() / reps;
+    if (ms) ms[v] = t;
+    if (t < tbest) {
+      tbest = t;
+      best = counts[v];
+    }
+  }
+  FL_TRY(set_zchunks(h, best));
+  FL_TRY(solve_impl(h, f, p));
+  FL_HIP(hipStreamSynchronize(s));
+  *chosen = h->zchunks;
+  return check_nonfinite(h);
 }
 
 int flutas_psolve_stats(const flutas_psolve *h, int *iters, double *relres) {

@@ -142,6 +142,29 @@ const char *flutas_psolve_stage_name(const flutas_psolve *h, int stage);
 int flutas_psolve_time_stage(flutas_psolve *h, const double *f, double *p, int stage,
                              int reps, double *ms);
 
+/* z-chunked slab passes (no counterpart in the reference, whose z solve
+ * walks whole columns, src/solver_gpu.f90:483-529 and
+ * src/solver_cpu.f90:187-223): the planes of each x-slab are eliminated in
+ * n chunks (1..8) by separate workgroups and joined by a reduced system of
+ * two unknowns per chunk and column, which gives plans with few x-slabs
+ * more workgroups than the GPU has CUs.  n = 0 restores the plan's default
+ * (doubling from 1 while nx * n < 512 and the chunks keep 32 planes or
+ * more); n = 1 walks whole slabs.  The count may come out lower than n for
+ * a short z.  Results equal the whole-slab walk's to rounding; the pivots
+ * stay the reference's bit for bit.  Changing the count frees the tables a
+ * recorded graph of this plan's solve refers to: record again after it.
+ * flutas_psolve_zchunks reads the count in use. */
+int flutas_psolve_set_zchunks(flutas_psolve *h, int n);
+int flutas_psolve_zchunks(const flutas_psolve *h, int *n);
+
+/* Autotuner of the single solve (no counterpart in the reference): times
+ * `reps` (1..100) solves f -> p at each chunk count 1, 2, 4, 8 that the
+ * grid allows, keeps the fastest and returns it in *chosen; ms (may be
+ * null) receives the mean milliseconds per solve for 1, 2, 4, 8 (-1 where
+ * the count was not possible).  p holds a solution afterwards. */
+int flutas_psolve_tune(flutas_psolve *h, const double *f, double *p, int reps, int *chosen,
+                       double *ms);
+
 /* ---- FP32 fields: FluTAS built with _SINGLE_PRECISION (rp = FP32,
  *      src/types.f90:6-12) ------------------------------------------------
  * The same entry points for FP32 fields and FP32 scalars, computing in

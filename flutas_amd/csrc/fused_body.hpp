@@ -43,10 +43,16 @@ constexpr int line_stride() {
 }
 
 // Lines per tile: the largest of 16, 8, 4, 2, 1 whose LDS tile fits
-// kTileBytes (two tiles per CU fit the 160 KiB of gfx950 LDS).
+// kTileBytes (two tiles per CU fit the 160 KiB of gfx950 LDS).  FP64
+// lines of 1024 points take 16 lines anyway: 131 KB plus 25.6 KB of staged
+// twiddles, one workgroup per CU, whose transposed rows then cover whole
+// 128-byte lines (1024x1024x512 FP64 solve: 16 lines and 8-plane slab
+// groups 9.18 ms; 8 lines / 4 planes, also one workgroup per CU, 10.40 ms;
+// 4 lines / 2 planes, two per CU, 10.82 ms).
 constexpr int kTileBytes = 72 * 1024;
 template <typename T, int NC>
 constexpr int tile_lines() {
+  if (NC == 512 && sizeof(T) == 8) return 16;
   constexpr int bytes = line_stride<T, 2 * NC>() * (int)sizeof(T);
   constexpr int l = kTileBytes / bytes;
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
@@ -129,9 +135,9 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 // Minimum waves per SIMD of a workgroup size: two 512-thread workgroups
 // per CU (4 waves per SIMD, at most 128 VGPRs each).  For 512-point lines
 // the LDS of two tiles or slabs (with their staged twiddles) fits a CU;
-// a 1024-point FP64 tile or slab group (8 lines or 2 x 4 planes of 1026
-// doubles, 65.7 KB, plus 25.6 KB of staged twiddles) does not, so those
-// run one workgroup (two waves per SIMD) per CU.
+// a 1024-point FP64 tile or slab group (16 lines or 2 x 8 planes of 1026
+// doubles, 131 KB, plus 25.6 KB of staged twiddles) fills one CU's LDS, so
+// those run one workgroup (two waves per SIMD) per CU.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
 // Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and for
@@ -722,10 +728,12 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
 }
 
 // P2/P3 plane groups: the two ping-pong tiles of a group stay within
-// kSlabBytes so that two slab workgroups fit one CU.
+// kSlabBytes so that two slab workgroups fit one CU; FP64 lines of 1024
+// points take 8 planes, one workgroup per CU (see tile_lines).
 constexpr int kSlabBytes = 72 * 1024;
 template <typename T, int NC>
 constexpr int group_planes() {
+  if (NC == 512 && sizeof(T) == 8) return 8;
   constexpr int bytes = 2 * line_stride<T, 2 * NC>() * (int)sizeof(T);
   constexpr int l = kSlabBytes / bytes;
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));

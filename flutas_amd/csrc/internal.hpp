@@ -95,15 +95,17 @@ int launch_lines(const LineTables &tab, int kind, const Tin *in, Tout *out,
                  const LineGeom &g, T scale, hipStream_t s);
 
 // ---- z tridiagonal solves ------------------------------------------------
+// Thomas solves of the line path (zsolve.hip): CK holds the elimination
+// checkpoints, thomas_checkpoint_doubles values.
+size_t thomas_checkpoint_doubles(int nx, int ny, int nz, bool periodic);
 template <typename T>
-int launch_thomas(T *W, double *D, const double *lamx, const double *lamy,
+int launch_thomas(T *W, double *CK, const double *lamx, const double *lamy,
                   const double *a, const double *b, const double *c, int nx,
                   int ny, int nz, int pin, hipStream_t s);
 template <typename T>
-int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
-                           const double *lamy, const double *a,
-                           const double *b, const double *c, int nx, int ny,
-                           int nz, int pin, hipStream_t s);
+int launch_thomas_periodic(T *W, double *CK, const double *lamx, const double *lamy,
+                           const double *a, const double *b, const double *c, int nx,
+                           int ny, int nz, int pin, hipStream_t s);
 
 // ---- fused four-pass pipeline --------------------------------------------
 // Layout of a work field, j contiguous: the x-slabs are grouped in blocks

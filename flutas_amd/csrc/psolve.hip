@@ -154,8 +154,7 @@ struct flutas_psolve {
   double *d_rhsbx = nullptr, *d_rhsby = nullptr, *d_rhsbz = nullptr;
   double *W64 = nullptr;   // spectral work field (FP64 mode)
   float *W32 = nullptr;    // spectral work field (refined mode)
-  double *D = nullptr;     // z elimination factors
-  double *P2 = nullptr;    // periodic-z second solution
+  double *TK = nullptr;    // z elimination checkpoints of the line path
   double *F = nullptr;     // copy of the right-hand side (refined mode)
   float *R32 = nullptr;    // residual (refined mode)
   // reduction partials: [0, nres) residual / absmax workgroups,
@@ -199,7 +198,7 @@ void release(flutas_psolve *h) {
   for (void *p : {(void *)h->d_lamx, (void *)h->d_lamy, (void *)h->d_a,
                   (void *)h->d_b, (void *)h->d_c, (void *)h->d_rhsbx,
                   (void *)h->d_rhsby, (void *)h->d_rhsbz, (void *)h->W64,
-                  (void *)h->W32, (void *)h->D, (void *)h->P2, (void *)h->F,
+                  (void *)h->W32, (void *)h->TK, (void *)h->F,
                   (void *)h->R32, (void *)h->part, (void *)h->CK, (void *)h->zc_mem})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -279,11 +278,11 @@ int zsolve(flutas_psolve *h, T *W, hipStream_t s) {
   const bool per = (h->cbc[4] == 'P' && h->cbc[5] == 'P');
   const int pin = (h->cbc[4] == 'N' && h->cbc[5] == 'N') || per;
   if (per)
-    return launch_thomas_periodic<T>(W, h->D, h->P2, h->d_lamx, h->d_lamy, h->d_a,
-                                     h->d_b, h->d_c, h->nx, h->ny, h->nz, pin, s);
+    return launch_thomas_periodic<T>(W, h->TK, h->d_lamx, h->d_lamy, h->d_a, h->d_b, h->d_c,
+                                     h->nx, h->ny, h->nz, pin, s);
   // a face-centred z grid with a D top face solves nz - 1 planes; the last
   // one is left as it is (src/solver_cpu.f90:71-77)
-  return launch_thomas<T>(W, h->D, h->d_lamx, h->d_lamy, h->d_a, h->d_b, h->d_c,
+  return launch_thomas<T>(W, h->TK, h->d_lamx, h->d_lamy, h->d_a, h->d_b, h->d_c,
                           h->nx, h->ny, h->nzs, pin, s);
 }
 
@@ -692,8 +691,8 @@ int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
       const size_t nck = std::max<size_t>(1, ngrp) * (cbc[4] == 'P' ? 2 : 1);
       if (e == hipSuccess) e = hipMalloc((void **)&h->CK, (size_t)h->nx * nck * h->ny * sizeof(double));
     } else {
-      if (e == hipSuccess) e = hipMalloc((void **)&h->D, N * sizeof(double));
-      if (e == hipSuccess && cbc[4] == 'P') e = hipMalloc((void **)&h->P2, N * sizeof(double));
+      const size_t ntk = thomas_checkpoint_doubles(h->nx, h->ny, h->nz, cbc[4] == 'P');
+      if (e == hipSuccess) e = hipMalloc((void **)&h->TK, ntk * sizeof(double));
     }
     if (e == hipSuccess) e = hipHostMalloc((void **)&h->nf_host, sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) {

@@ -12,11 +12,19 @@
 
 namespace flutas {
 
-// Non-periodic z (src/solver_cpu.f90:117-145, :187-223).  D is an
-// nx*ny*nz FP64 scratch for the elimination factors d(k).
+// The factors d(k) are not stored for every plane: the forward sweep
+// keeps one checkpoint (the d before the group) every kZsGroup planes and
+// the backward sweep recomputes each group's factors from it with the same
+// operations, so they are the forward sweep's bit for bit.  ck holds
+// nx*ny*zs_groups(n) values (twice that for periodic z, whose second
+// right-hand side keeps checkpoints of its own).
+constexpr int kZsGroup = 8;
+__host__ __device__ constexpr int zs_groups(int n) { return (n + kZsGroup - 1) / kZsGroup; }
+
+// Non-periodic z (src/solver_cpu.f90:117-145, :187-223).
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_thomas(T *__restrict__ W, double *__restrict__ D,
+    k_thomas(T *__restrict__ W, double *__restrict__ CK,
              const double *__restrict__ lamx, const double *__restrict__ lamy,
              const double *__restrict__ a, const double *__restrict__ b,
              const double *__restrict__ c, int nx, int ny, int n, int pin) {
@@ -27,41 +35,71 @@ __global__ void __launch_bounds__(256)
   const double lam = lamx[i] + lamy[j];
   const long long st = (long long)nx * ny;
   T *p = W + i + (long long)nx * j;
-  double *d = D + i + (long long)nx * j;
-  double z = 1.0 / (b[0] + lam);
-  double dprev = c[0] * z;
-  d[0] = dprev;
-  double pprev = (double)p[0] * z;
-  p[0] = (T)pprev;
-  for (int l = 1; l < n - 1; ++l) {
-    const double bb = b[l] + lam;
-    z = 1.0 / (bb - a[l] * dprev);
-    dprev = c[l] * z;
-    d[l * st] = dprev;
-    pprev = ((double)p[l * st] - a[l] * pprev) * z;
+  double *ck = CK + i + (long long)nx * j;
+  double dprev = 0.0, pprev = 0.0;
+  for (int l = 0; l < n; ++l) {
+    if (l % kZsGroup == 0) ck[(l / kZsGroup) * st] = dprev;
+    const double y = (double)p[l * st];
+    if (l == 0) {
+      const double z = 1.0 / (b[0] + lam);
+      dprev = c[0] * z;
+      pprev = y * z;
+    } else if (l < n - 1) {
+      const double bb = b[l] + lam;
+      const double z = 1.0 / (bb - a[l] * dprev);
+      dprev = c[l] * z;
+      pprev = (y - a[l] * pprev) * z;
+    } else {
+      const double z = (b[n - 1] + lam) - a[n - 1] * dprev;
+      // the singular column of an NN z-problem (lambda == 0) is pinned to
+      // p(n) = 0, the reference's zero-pivot branch
+      // (src/solver_cpu.f90:209-214)
+      if (z != 0.0 && !(pin && lam == 0.0))
+        pprev = (y - a[n - 1] * pprev) / z;
+      else
+        pprev = 0.0;
+    }
     p[l * st] = (T)pprev;
   }
-  z = (b[n - 1] + lam) - a[n - 1] * dprev;
-  // the singular column of an NN z-problem (lambda == 0) is pinned to
-  // p(n) = 0, the reference's zero-pivot branch (src/solver_cpu.f90:209-214)
-  if (z != 0.0 && !(pin && lam == 0.0))
-    pprev = ((double)p[(n - 1) * st] - a[n - 1] * pprev) / z;
-  else
-    pprev = 0.0;
-  p[(n - 1) * st] = (T)pprev;
-  for (int l = n - 2; l >= 0; --l) {
-    pprev = (double)p[l * st] - d[l * st] * pprev;
-    p[l * st] = (T)pprev;
+  // back substitution, group by group from the top
+  for (int g = zs_groups(n) - 1; g >= 0; --g) {
+    const int k0 = g * kZsGroup;
+    double d[kZsGroup];
+    double dp = ck[g * st];
+#pragma unroll
+    for (int l = 0; l < kZsGroup; ++l) {
+      const int k = k0 + l;
+      if (k < n - 1) {
+        if (k == 0) {
+          dp = c[0] * (1.0 / (b[0] + lam));
+        } else {
+          const double bb = b[k] + lam;
+          dp = c[k] * (1.0 / (bb - a[k] * dp));
+        }
+      }
+      d[l] = dp;
+    }
+#pragma unroll
+    for (int l = kZsGroup - 1; l >= 0; --l) {
+      const int k = k0 + l;
+      if (k >= n - 1) continue;
+      pprev = (double)p[k * st] - d[l] * pprev;
+      p[k * st] = (T)pprev;
+    }
   }
 }
 
-// Periodic z (src/solver_cpu.f90:147-185): two Thomas solves of order n-1
-// sharing the factors d, then the rank-one closure for p(n).  P2 holds the
-// second solution.  pin: the lambda == 0 column (singular) takes p(n) = 0.
+// Periodic z (src/solver_cpu.f90:147-185): the first m = n-1 rows are
+// eliminated for the field (p1) and for r2 = (-a(1), 0, ..., 0, -c(m)) (p2)
+// with the same factors; p1(1) and p2(1) come out of the forward sweep as
+// sums of g(k) p'(k), g(k) = prod_{k'<k} (-d(k')), so the closing row
+// p(n) is formed there, and one backward sweep writes p1 + p2 p(n), with
+// p2's forward values recomputed from their own checkpoints.  pin: the
+// lambda == 0 column (singular) takes p(n) = 0.
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_thomas_periodic(T *__restrict__ W, double *__restrict__ D,
-                      double *__restrict__ P2, const double *__restrict__ lamx,
+    k_thomas_periodic(T *__restrict__ W, double *__restrict__ CK,
+                      const double *__restrict__ lamx,
                       const double *__restrict__ lamy,
                       const double *__restrict__ a,
                       const double *__restrict__ b,
@@ -74,81 +112,121 @@ __global__ void __launch_bounds__(256)
   const double lam = lamx[i] + lamy[j];
   const long long st = (long long)nx * ny;
   T *p = W + i + (long long)nx * j;
-  double *d = D + i + (long long)nx * j;
-  double *q = P2 + i + (long long)nx * j;
+  double *ckd = CK + i + (long long)nx * j;
+  double *ckq = ckd + st * zs_groups(n);
   const int m = n - 1;  // order of the two reduced systems
-  // forward elimination of p1 (in W) and p2 (in P2)
-  double z = 1.0 / (b[0] + lam);
-  double dprev = c[0] * z;
-  d[0] = dprev;
-  double p1 = (double)p[0] * z;
-  p[0] = (T)p1;
-  // p2 starts as (-a(1), 0, ..., 0, -c(n-1))
-  double p2 = (m == 1 ? -c[0] : -a[0]) * z;
-  q[0] = p2;
-  for (int l = 1; l < m - 1; ++l) {
-    const double bb = b[l] + lam;
-    z = 1.0 / (bb - a[l] * dprev);
-    dprev = c[l] * z;
-    d[l * st] = dprev;
-    p1 = ((double)p[l * st] - a[l] * p1) * z;
+  double dprev = 0.0, p1 = 0.0, q = 0.0, s1 = 0.0, s2 = 0.0, g = 1.0;
+  for (int l = 0; l < m; ++l) {
+    if (l % kZsGroup == 0) {
+      ckd[(l / kZsGroup) * st] = dprev;
+      ckq[(l / kZsGroup) * st] = q;
+    }
+    const double y = (double)p[l * st];
+    // r2 = (-a(1), 0, ..., 0, -c(m))
+    const double r2 = (l == m - 1) ? -c[m - 1] : ((l == 0) ? -a[0] : 0.0);
+    if (l == 0) {
+      const double z = 1.0 / (b[0] + lam);
+      dprev = c[0] * z;
+      p1 = y * z;
+      q = r2 * z;
+    } else if (l < m - 1) {
+      const double bb = b[l] + lam;
+      const double z = 1.0 / (bb - a[l] * dprev);
+      dprev = c[l] * z;
+      p1 = (y - a[l] * p1) * z;
+      q = (r2 - a[l] * q) * z;
+    } else {
+      const double z = (b[m - 1] + lam) - a[m - 1] * dprev;
+      if (z != 0.0) {
+        p1 = (y - a[m - 1] * p1) / z;
+        q = (r2 - a[m - 1] * q) / z;
+      } else {
+        p1 = 0.0;
+        q = 0.0;
+      }
+    }
+    s1 = s1 + g * p1;
+    s2 = s2 + g * q;
+    g = -(dprev * g);
     p[l * st] = (T)p1;
-    p2 = (0.0 - a[l] * p2) * z;
-    q[l * st] = p2;
   }
-  z = (b[m - 1] + lam) - a[m - 1] * dprev;
-  const double rhs2 = -c[m - 1];
-  if (z != 0.0) {
-    p1 = ((double)p[(m - 1) * st] - a[m - 1] * p1) / z;
-    p2 = (rhs2 - a[m - 1] * p2) / z;
-  } else {
-    p1 = 0.0;
-    p2 = 0.0;
-  }
-  p[(m - 1) * st] = (T)p1;
-  q[(m - 1) * st] = p2;
-  for (int l = m - 2; l >= 0; --l) {
-    const double dl = d[l * st];
-    p1 = (double)p[l * st] - dl * p1;
-    p[l * st] = (T)p1;
-    p2 = q[l * st] - dl * p2;
-    q[l * st] = p2;
-  }
-  // closure (p1, p2 now hold element 1 of each solution)
-  const double p1n1 = (double)p[(m - 1) * st], p2n1 = q[(m - 1) * st];
+  // closure with p1(1) = s1, p2(1) = s2, p1(m) = p1, p2(m) = q
   double pn;
   if (pin && lam == 0.0)
     pn = 0.0;
   else
-    pn = ((double)p[(n - 1) * st] - c[n - 1] * p1 - a[n - 1] * p1n1) /
-         ((b[n - 1] + lam) + c[n - 1] * p2 + a[n - 1] * p2n1);
-  p[(n - 1) * st] = (T)pn;
-  for (int l = 0; l < m; ++l)
-    p[l * st] = (T)((double)p[l * st] + q[l * st] * pn);
+    pn = ((double)p[m * st] - c[n - 1] * s1 - a[n - 1] * p1) /
+         ((b[n - 1] + lam) + c[n - 1] * s2 + a[n - 1] * q);
+  p[m * st] = (T)pn;
+  double p1n = 0.0, p2n = 0.0;
+  for (int gr = zs_groups(m) - 1; gr >= 0; --gr) {
+    const int k0 = gr * kZsGroup;
+    double d[kZsGroup], qq[kZsGroup];
+    double dp = ckd[gr * st], qp = ckq[gr * st];
+#pragma unroll
+    for (int l = 0; l < kZsGroup; ++l) {
+      const int k = k0 + l;
+      if (k < m) {
+        const double r2 = (k == m - 1) ? -c[m - 1] : ((k == 0) ? -a[0] : 0.0);
+        if (k == 0) {
+          const double z = 1.0 / (b[0] + lam);
+          dp = c[0] * z;
+          qp = r2 * z;
+        } else if (k < m - 1) {
+          const double bb = b[k] + lam;
+          const double z = 1.0 / (bb - a[k] * dp);
+          dp = c[k] * z;
+          qp = (r2 - a[k] * qp) * z;
+        } else {
+          const double z = (b[m - 1] + lam) - a[m - 1] * dp;
+          qp = (z != 0.0) ? (r2 - a[m - 1] * qp) / z : 0.0;
+        }
+      }
+      d[l] = dp;
+      qq[l] = qp;
+    }
+#pragma unroll
+    for (int l = kZsGroup - 1; l >= 0; --l) {
+      const int k = k0 + l;
+      if (k >= m) continue;
+      const double pk = (double)p[k * st];
+      double x1, x2;
+      if (k == m - 1) {
+        x1 = pk;
+        x2 = qq[l];
+      } else {
+        x1 = pk - d[l] * p1n;
+        x2 = qq[l] - d[l] * p2n;
+      }
+      p1n = x1;
+      p2n = x2;
+      p[k * st] = (T)(x1 + x2 * pn);
+    }
+  }
 }
 
 template <typename T>
-int launch_thomas(T *W, double *D, const double *lamx, const double *lamy,
+int launch_thomas(T *W, double *CK, const double *lamx, const double *lamy,
                   const double *a, const double *b, const double *c, int nx,
                   int ny, int nz, int pin, hipStream_t s) {
   if (nz < 2) return fail(FLUTAS_EINVAL, "nz must be >= 2");
   dim3 blk(std::min(256, ((nx + 63) / 64) * 64));
   dim3 grid((nx + blk.x - 1) / blk.x, ny);
-  hipLaunchKernelGGL(k_thomas<T>, grid, blk, 0, s, W, D, lamx, lamy, a, b, c,
+  hipLaunchKernelGGL(k_thomas<T>, grid, blk, 0, s, W, CK, lamx, lamy, a, b, c,
                      nx, ny, nz, pin);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
 }
 
 template <typename T>
-int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
+int launch_thomas_periodic(T *W, double *CK, const double *lamx,
                            const double *lamy, const double *a,
                            const double *b, const double *c, int nx, int ny,
                            int nz, int pin, hipStream_t s) {
   if (nz < 3) return fail(FLUTAS_EINVAL, "periodic z needs nz >= 3");
   dim3 blk(std::min(256, ((nx + 63) / 64) * 64));
   dim3 grid((nx + blk.x - 1) / blk.x, ny);
-  hipLaunchKernelGGL(k_thomas_periodic<T>, grid, blk, 0, s, W, D, P2, lamx,
+  hipLaunchKernelGGL(k_thomas_periodic<T>, grid, blk, 0, s, W, CK, lamx,
                      lamy, a, b, c, nx, ny, nz, pin);
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -156,7 +234,11 @@ int launch_thomas_periodic(T *W, double *D, double *P2, const double *lamx,
 
 template int launch_thomas<double>(double *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
 template int launch_thomas<float>(float *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
-template int launch_thomas_periodic<double>(double *, double *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
-template int launch_thomas_periodic<float>(float *, double *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
+template int launch_thomas_periodic<double>(double *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
+template int launch_thomas_periodic<float>(float *, double *, const double *, const double *, const double *, const double *, const double *, int, int, int, int, hipStream_t);
+
+size_t thomas_checkpoint_doubles(int nx, int ny, int nz, bool periodic) {
+  return (size_t)nx * ny * zs_groups(nz) * (periodic ? 2 : 1);
+}
 
 }  // namespace flutas

@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cmath>
 #include <mutex>
+#include <vector>
 
 namespace flutas {
 
@@ -1022,6 +1023,186 @@ static int boundp_impl(const char cbc[6], const int n[3], const T bc[6],
   if (cbc[4] != 'P') { FL_TRY(face(cbc[4], 0, 2, bc[4], dzc_0)); FL_TRY(face(cbc[5], 1, 2, bc[5], dzc_n)); }
   return FLUTAS_OK;
 }
+// ---- bounduvw in one launch ----------------------------------------------
+// The face launches of bounduvw (updthalo in y and z, then the x, y, z
+// faces, each side in turn) as one kernel.  A face stage writes a halo
+// position from one value of its own line (a copy, a constant, or
+// A + sgn * value, sgn = +-1), so the final value of a point is a chain: its
+// last writer's rule applied to the source point's value at the stage
+// before, and so on, ending at a constant or at a point no stage writes.
+// Each thread walks that chain for its own point and applies the rules
+// from the innermost out, the order and operations of the face launches,
+// so the result is theirs bit for bit.  A chain only moves a coordinate
+// from the halo into the line's interior and, because a stage's write set
+// depends only on the coordinate along its direction, every stage writing
+// a halo coordinate of the end point also wrote the start point earlier in
+// the chain: the end point is never written by any stage.  So each thread
+// writes only its own point and reads only points nobody writes.  Used
+// when every n > 2 nh (so that a source never lands in the opposite halo).
+constexpr int kUvwStages = 8;
+template <typename T>
+struct UvwStage {
+  int dir, side;
+  char t[3];  // rule per field ('P' both sides, 'D', 'N'); 0: not touched
+  T f0[3];
+  T dr[3][kMaxHalo];
+};
+template <typename T>
+struct UvwArgs {
+  T *p[3];
+  UvwStage<T> st[kUvwStages];
+  int nst, nh;
+  int n[3];
+  long long cnt[3];  // points of the direction slabs of the shell
+};
+
+// Rule of stage S for field c at position pos along S.dir: 0 not written,
+// 1 constant A, 2 copy of src, 3 A + sg * (value at src).
+template <typename T>
+__device__ __forceinline__ int uvw_rule(const UvwStage<T> &S, int c, int pos, int nd, int nh,
+                                        T *A, T *sg, int *src) {
+#pragma clang fp contract(off)
+  const char t = S.t[c];
+  if (!t) return 0;
+  if (t == 'P') {
+    if (pos <= 0 && pos >= 1 - nh) {
+      *src = nd + pos;
+      return 2;
+    }
+    if (pos >= nd + 1 && pos <= nd + nh) {
+      *src = pos - nd;
+      return 2;
+    }
+    return 0;
+  }
+  const T f0 = S.f0[c];
+  if (c != S.dir) {
+    // cell-centred component: f + sgn * L, f = 2 value (D) or -+dr value (N)
+    int q;
+    if (S.side == 0) {
+      if (pos > 0 || pos < 1 - nh) return 0;
+      q = -pos;
+      *src = 1 + q;
+      *A = (t == 'D') ? T(2.0) * f0 : -S.dr[c][q] * f0;
+    } else {
+      if (pos < nd + 1 || pos > nd + nh) return 0;
+      q = pos - nd - 1;
+      *src = nd - q;
+      *A = (t == 'D') ? T(2.0) * f0 : S.dr[c][q] * f0;
+    }
+    *sg = (t == 'D') ? -T(1.0) : T(1.0);
+    return 3;
+  }
+  if (t == 'D') {
+    // face-normal wall: the value; the outer upper layer copies nd - nh
+    if (S.side == 0) {
+      if (pos > 0 || pos < 1 - nh) return 0;
+      *A = f0;
+      return 1;
+    }
+    if (pos >= nd && pos <= nd + nh - 1) {
+      *A = f0;
+      return 1;
+    }
+    if (pos == nd + nh) {
+      *src = nd - nh;
+      return 2;
+    }
+    return 0;
+  }
+  // face-normal N: 1.0 f + L (2.0 f + L on the outer upper layer)
+  *sg = T(1.0);
+  if (S.side == 0) {
+    if (pos > 0 || pos < 1 - nh) return 0;
+    const int q = -pos;
+    *A = T(1.0) * (-S.dr[c][q] * f0);
+    *src = 1 + q;
+    return 3;
+  }
+  if (pos >= nd && pos <= nd + nh - 1) {
+    const int r = pos - nd;
+    *A = T(1.0) * (S.dr[c][r] * f0);
+    *src = nd - 1 - r;
+    return 3;
+  }
+  if (pos == nd + nh) {
+    *A = T(2.0) * (S.dr[c][nh - 1] * f0);
+    *src = nd - nh;
+    return 3;
+  }
+  return 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bounduvw_one(UvwArgs<T> a) {
+#pragma clang fp contract(off)
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  int d = 0;
+  while (d < 3 && t >= a.cnt[d]) {
+    t -= a.cnt[d];
+    ++d;
+  }
+  if (d == 3) return;
+  const int nh = a.nh;
+  // slab d: positions 1-nh..0 and n_d..n_d+nh along d; along e < d the
+  // interior positions 1..n_e-1 (the rest belongs to slab e), along e > d
+  // every position
+  int pos[3];
+  const int L = (int)(t % (2 * nh + 1));
+  t /= (2 * nh + 1);
+  pos[d] = (L < nh) ? (1 - nh + L) : (a.n[d] + (L - nh));
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    if (e == d) continue;
+    const int len = (e < d) ? a.n[e] - 1 : a.n[e] + 2 * nh;
+    pos[e] = ((e < d) ? 1 : 1 - nh) + (int)(t % len);
+    t /= len;
+  }
+  const long long ex = a.n[0] + 2LL * nh, exy = ex * (a.n[1] + 2LL * nh);
+  auto at = [&](const int *q) {
+    return (q[0] - 1 + nh) + ex * (q[1] - 1 + nh) + exy * (q[2] - 1 + nh);
+  };
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    int cur[3] = {pos[0], pos[1], pos[2]};
+    T As[kUvwStages], Ss[kUvwStages];
+    int nr = 0, s = a.nst - 1;
+    bool written = false;
+    T v = T(0.0);
+    for (;;) {
+      int kind = 0, src = 0, sw = -1;
+      T A = T(0.0), sg = T(0.0);
+      for (; s >= 0; --s) {
+        const int dd = a.st[s].dir;
+        kind = uvw_rule(a.st[s], c, cur[dd], a.n[dd], nh, &A, &sg, &src);
+        if (kind) {
+          sw = s;
+          break;
+        }
+      }
+      if (sw < 0) {
+        v = a.p[c][at(cur)];
+        break;
+      }
+      written = true;
+      if (kind == 1) {
+        v = A;
+        break;
+      }
+      if (kind == 3) {
+        As[nr] = A;
+        Ss[nr] = sg;
+        ++nr;
+      }
+      cur[a.st[sw].dir] = src;
+      s = sw - 1;
+    }
+    if (!written) continue;
+    for (int r = nr - 1; r >= 0; --r) v = As[r] + Ss[r] * v;
+    a.p[c][at(pos)] = v;
+  }
+}
+
 template <typename T>
 static int bounduvw_impl(const char cbc[18], const int n[3], const T bc[18],
                     int nh_d, int nh_u, const int isoutflow[6],
@@ -1049,11 +1230,11 @@ static int bounduvw_impl(const char cbc[18], const int n[3], const T bc[18],
   T *fld[3] = {u, v, w};
   const int nz = n[2];
   auto Z = [&](const T *a, int k) { return a[k + nh_d - 1]; };
+  // the face stages in the reference's order, launched one by one or, when
+  // every n > 2 nh, as one k_bounduvw_one launch
+  std::vector<FaceArgs<T>> stages;
   auto launch = [&](FaceArgs<T> &a) -> int {
-    const int t1 = (a.dir == 0) ? 1 : 0, t2 = (a.dir == 2) ? 1 : 2;
-    const int e1 = n[t1] + 2 * nh_u, e2 = n[t2] + 2 * nh_u;
-    hipLaunchKernelGGL(k_set_bc_face<T>, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
-    FL_HIP(hipGetLastError());
+    stages.push_back(a);
     return FLUTAS_OK;
   };
   auto face = [&](int dir, int side) {
@@ -1096,6 +1277,44 @@ static int bounduvw_impl(const char cbc[18], const int n[3], const T bc[18],
       }
       if (a.f[0].ctype || a.f[1].ctype || a.f[2].ctype) FL_TRY(launch(a));
     }
+  const bool one = n[0] > 2 * nh_u && n[1] > 2 * nh_u && n[2] > 2 * nh_u &&
+                   (int)stages.size() <= kUvwStages;
+  if (one && !stages.empty()) {
+    UvwArgs<T> A{};
+    for (int c = 0; c < 3; ++c) A.p[c] = fld[c];
+    A.nst = (int)stages.size();
+    A.nh = nh_u;
+    for (int q = 0; q < 3; ++q) A.n[q] = n[q];
+    for (int k = 0; k < A.nst; ++k) {
+      const FaceArgs<T> &F = stages[k];
+      UvwStage<T> &S = A.st[k];
+      S.dir = F.dir;
+      S.side = F.side;
+      for (int c = 0; c < 3; ++c) {
+        S.t[c] = F.f[c].ctype;
+        S.f0[c] = F.f[c].value;
+        for (int q = 0; q < kMaxHalo; ++q) S.dr[c][q] = F.f[c].dr[q];
+      }
+    }
+    long long total = 0;
+    for (int d = 0; d < 3; ++d) {
+      long long m = 2 * nh_u + 1;
+      for (int e = 0; e < 3; ++e)
+        if (e != d) m *= (e < d) ? n[e] - 1 : n[e] + 2 * nh_u;
+      A.cnt[d] = m;
+      total += m;
+    }
+    hipLaunchKernelGGL(k_bounduvw_one<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       A);
+    FL_HIP(hipGetLastError());
+  } else {
+    for (FaceArgs<T> &a : stages) {
+      const int t1 = (a.dir == 0) ? 1 : 0, t2 = (a.dir == 2) ? 1 : 2;
+      const int e1 = n[t1] + 2 * nh_u, e2 = n[t2] + 2 * nh_u;
+      hipLaunchKernelGGL(k_set_bc_face<T>, dim3((e1 + 255) / 256, e2), dim3(256), 0, s, a);
+      FL_HIP(hipGetLastError());
+    }
+  }
   // outflow faces
   bool need_dzf = false;
   for (int d = 0; d < 2; ++d) need_dzf = need_dzf || isoutflow[2 * d] || isoutflow[2 * d + 1];

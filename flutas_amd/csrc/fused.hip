@@ -118,7 +118,9 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
 // walking down group by group (factors recomputed from the stored values),
 // the second solution p2 at every chunk start and the closing row's
 // denominator (b(m) + lam) + c(m) p2(0) + a(m) p2(m-1).
+// The pivot reciprocals are those of the chain the tables serve (zrcp<T>).
 constexpr int kZcMaxGroup = 16;
+template <typename T>
 __global__ void __launch_bounds__(256)
     k_zchunk_tables(double *__restrict__ ckd, double *__restrict__ ckm, double *__restrict__ ckq,
                     double *__restrict__ tab, double *__restrict__ den, int nx, int ny, int nz,
@@ -152,13 +154,13 @@ __global__ void __launch_bounds__(256)
       const double r2 = per ? ((k == msys - 1) ? -c[msys - 1] : ((k == 0) ? -a[0] : 0.0)) : 0.0;
       double dl = 0.0;
       if (k == 0) {
-        const double z = 1.0 / (b[0] + lam);
+        const double z = zrcp<T>(b[0] + lam);
         dp = c[0] * z;
         mp = 0.0;
         qp = r2 * z;
         dl = dp;
       } else if (k < msys - 1) {
-        const double z = 1.0 / ((b[k] + lam) - a[k] * dp);
+        const double z = zrcp<T>((b[k] + lam) - a[k] * dp);
         dp = c[k] * z;
         mp = (-a[k] * mp) * z;
         qp = (r2 - a[k] * qp) * z;
@@ -194,11 +196,11 @@ __global__ void __launch_bounds__(256)
       if (l < gp && k < m) {
         const double r2 = (k == m - 1) ? -c[m - 1] : ((k == 0) ? -a[0] : 0.0);
         if (k == 0) {
-          const double z = 1.0 / (b[0] + lam);
+          const double z = zrcp<T>(b[0] + lam);
           d2 = c[0] * z;
           q2 = r2 * z;
         } else if (k < m - 1) {
-          const double z = 1.0 / ((b[k] + lam) - a[k] * d2);
+          const double z = zrcp<T>((b[k] + lam) - a[k] * d2);
           d2 = c[k] * z;
           q2 = (r2 - a[k] * q2) * z;
         } else {
@@ -428,9 +430,9 @@ size_t zchunk_doubles(int nx, int ny, int ngrp, int nchunk, bool per) {
 }
 
 int build_zchunk_tables(double *mem, ZChunks *zc, int nx, int ny, int nz, int msys, int gp,
-                        int cpl, int nchunk, bool per, int pin, const double *lamx,
-                        const double *lamy, const double *a, const double *b, const double *c,
-                        hipStream_t s) {
+                        int cpl, int nchunk, bool per, int pin, int elem_bytes,
+                        const double *lamx, const double *lamy, const double *a,
+                        const double *b, const double *c, hipStream_t s) {
   if (gp < 1 || gp > kZcMaxGroup || cpl % gp != 0 || nchunk < 2 || nchunk > kMaxZChunks ||
       (nchunk - 1) * cpl > nz - 2)
     return fail(FLUTAS_EINVAL, "z chunks: bad chunk shape");
@@ -442,9 +444,14 @@ int build_zchunk_tables(double *mem, ZChunks *zc, int nx, int ny, int nz, int ms
   double *side = den + col;
   const int threads = 256;
   const long long blocks = ((long long)col + threads - 1) / threads;
-  hipLaunchKernelGGL(k_zchunk_tables, dim3((unsigned)blocks), dim3(threads), 0, s, ckd, ckm, ckq,
-                     tab, den, nx, ny, nz, msys, gp, cpl, nchunk, per ? 1 : 0, pin, lamx, lamy,
-                     a, b, c);
+  if (elem_bytes == 8)
+    hipLaunchKernelGGL(k_zchunk_tables<double>, dim3((unsigned)blocks), dim3(threads), 0, s, ckd,
+                       ckm, ckq, tab, den, nx, ny, nz, msys, gp, cpl, nchunk, per ? 1 : 0, pin,
+                       lamx, lamy, a, b, c);
+  else
+    hipLaunchKernelGGL(k_zchunk_tables<float>, dim3((unsigned)blocks), dim3(threads), 0, s, ckd,
+                       ckm, ckq, tab, den, nx, ny, nz, msys, gp, cpl, nchunk, per ? 1 : 0, pin,
+                       lamx, lamy, a, b, c);
   FL_HIP(hipGetLastError());
   *zc = ZChunks{nchunk, cpl, ngrp, ckd, ckm, ckq, tab, den, side};
   return FLUTAS_OK;

@@ -31,7 +31,8 @@
 // d, m (and for periodic z the second solution p2 of the cyclic system) do
 // not depend on the right-hand side: k_zchunk_tables computes them once per
 // plan, one thread per column, with the same operations as the passes, so
-// the pivots d(k) stay the reference's bit for bit; P3 recomputes the d, m
+// the pivots d(k) of FP64 chains stay the reference's bit for bit (FP32
+// chains take the reciprocal zrcp); P3 recomputes the d, m
 // (and q2) of each plane group from the tables' values at the group start.
 // For periodic z the closing plane x(m), m = nz - 1, is
 //   (y(m) - c(m) p1(0) - a(m) p1(m-1)) / (b(m) + lam + c(m) p2(0) + a(m) p2(m-1))
@@ -168,12 +169,12 @@ __device__ __forceinline__ void yz_fwd_chunk(T *__restrict__ Wt, WStride ws, con
           pv = y;
           if (PER) ym[u] = y;
         } else if (k == 0) {
-          const double z = 1.0 / (bk + lam[u]);
+          const double z = zrcp<T>(bk + lam[u]);
           dprev[u] = cc * z;
           dk = dprev[u];
           pv = y * z;
         } else if (k < msys - 1) {
-          const double z = 1.0 / ((bk + lam[u]) - ak * dprev[u]);
+          const double z = zrcp<T>((bk + lam[u]) - ak * dprev[u]);
           dprev[u] = cc * z;
           dk = dprev[u];
           pv = (y - ak * pprev[u]) * z;
@@ -346,13 +347,13 @@ __device__ __forceinline__ void zy_bwd_chunk(T *__restrict__ Wt, WStride ws, con
         if (l < np && k < msys) {
           const double r2 = PER ? ((k == msys - 1) ? -c[msys - 1] : ((k == 0) ? -a[0] : 0.0)) : 0.0;
           if (k == 0) {
-            const double z = 1.0 / (b[0] + lam[u]);
+            const double z = zrcp<T>(b[0] + lam[u]);
             dp = c[0] * z;
             mp = 0.0;
             if (PER) qp = r2 * z;
             dl = dp;
           } else if (k < msys - 1) {
-            const double z = 1.0 / ((b[k] + lam[u]) - a[k] * dp);
+            const double z = zrcp<T>((b[k] + lam[u]) - a[k] * dp);
             dp = c[k] * z;
             mp = (-a[k] * mp) * z;
             if (PER) qp = (r2 - a[k] * qp) * z;

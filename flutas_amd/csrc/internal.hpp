@@ -166,9 +166,9 @@ __host__ __device__ constexpr int zc_side_slots(bool per) { return per ? 3 : 2; 
 // 3 ngrp + nchunk tab slots + 1 + nchunk side slots values per column).
 size_t zchunk_doubles(int nx, int ny, int ngrp, int nchunk, bool per);
 int build_zchunk_tables(double *mem, ZChunks *zc, int nx, int ny, int nz, int msys, int gp,
-                        int cpl, int nchunk, bool per, int pin, const double *lamx,
-                        const double *lamy, const double *a, const double *b, const double *c,
-                        hipStream_t s);
+                        int cpl, int nchunk, bool per, int pin, int elem_bytes,
+                        const double *lamx, const double *lamy, const double *a,
+                        const double *b, const double *c, hipStream_t s);
 template <typename T>
 int launch_yz_fwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZChunks &zc,
                      int nx, int nz, int msys, const double *lamx, const double *lamy,

@@ -336,8 +336,8 @@ int set_zchunks(flutas_psolve *h, int n) {
   const size_t nd = zchunk_doubles(h->nx, h->ny, ngrp, nch, per != 0);
   FL_HIP(hipMalloc((void **)&h->zc_mem, nd * sizeof(double)));
   const int rc = build_zchunk_tables(h->zc_mem, &h->zc, h->nx, h->ny, nz, per ? nz - 1 : h->nzs,
-                                     gp, cpl, nch, per != 0, pin, h->d_lamx, h->d_lamy, h->d_a,
-                                     h->d_b, h->d_c, nullptr);
+                                     gp, cpl, nch, per != 0, pin, elem, h->d_lamx, h->d_lamy,
+                                     h->d_a, h->d_b, h->d_c, nullptr);
   if (rc == FLUTAS_OK && hipDeviceSynchronize() != hipSuccess)
     return fail(FLUTAS_EDEVICE, "z chunk tables");
   if (rc) {

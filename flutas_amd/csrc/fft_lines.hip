@@ -442,14 +442,14 @@ LinePlan<double> LineTables::dplan() const {
   LinePlan<double> p{};
   p.n = n; p.N = N; p.nst = nst;
   for (int i = 0; i < kMaxStages; ++i) p.radix[i] = radix[i];
-  p.tw = d_tw; p.twh = d_twh; p.twq = d_twq; p.twe = d_twe;
+  p.tw = d_tw; p.twh = d_twh; p.twq = d_twq; p.twe = d_twe; p.t4 = d_t4;
   return p;
 }
 LinePlan<float> LineTables::fplan() const {
   LinePlan<float> p{};
   p.n = n; p.N = N; p.nst = nst;
   for (int i = 0; i < kMaxStages; ++i) p.radix[i] = radix[i];
-  p.tw = f_tw; p.twh = f_twh; p.twq = f_twq; p.twe = f_twe;
+  p.tw = f_tw; p.twh = f_twh; p.twq = f_twq; p.twe = f_twe; p.t4 = f_t4;
   return p;
 }
 
@@ -461,7 +461,8 @@ int LineTables::build(int n_) {
   nst = (N == 1) ? 0 : factorize(N, radix);
   if (nst < 0) return fail(FLUTAS_EUNSUPPORTED, "n/2 has a prime factor > 1021");
   const long double pi = 3.14159265358979323846264338327950288L;
-  std::vector<double> tw(2 * N), twh(2 * (N + 1)), twq(2 * (n + 1)), twe(2 * N);
+  std::vector<double> tw(2 * N), twh(2 * (N + 1)), twq(2 * (n + 1)), twe(2 * N),
+      t4(4 * (N + 1));
   for (int m = 0; m < N; ++m) {
     long double a = 2.0L * pi * m / N;
     tw[2 * m] = (double)cosl(a);
@@ -482,6 +483,13 @@ int LineTables::build(int n_) {
     twe[2 * m] = (double)cosl(a);
     twe[2 * m + 1] = (double)-sinl(a);
   }
+  for (int m = 0; m <= N; ++m) {
+    const long double a = pi * (4.0L * m + 1.0L) / (4.0L * n), b = pi * m / (long double)n;
+    t4[2 * m] = (double)cosl(a);
+    t4[2 * m + 1] = (double)-sinl(a);
+    t4[2 * (N + 1) + 2 * m] = (double)cosl(b);
+    t4[2 * (N + 1) + 2 * m + 1] = (double)-sinl(b);
+  }
   auto up = [](const std::vector<double> &h, double **dd, float **df) -> int {
     FL_HIP(hipMalloc(dd, h.size() * sizeof(double)));
     FL_HIP(hipMemcpy(*dd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -494,16 +502,17 @@ int LineTables::build(int n_) {
   FL_TRY(up(twh, &d_twh, &f_twh));
   FL_TRY(up(twq, &d_twq, &f_twq));
   FL_TRY(up(twe, &d_twe, &f_twe));
+  FL_TRY(up(t4, &d_t4, &f_t4));
   return FLUTAS_OK;
 }
 
 void LineTables::release() {
   for (void *p : {(void *)d_tw, (void *)d_twh, (void *)d_twq, (void *)f_tw,
                   (void *)f_twh, (void *)f_twq, (void *)d_ptw, (void *)f_ptw,
-                  (void *)d_twe, (void *)f_twe})
+                  (void *)d_twe, (void *)f_twe, (void *)d_t4, (void *)f_t4})
     if (p) (void)hipFree(p);
-  d_tw = d_twh = d_twq = d_twe = nullptr;
-  f_tw = f_twh = f_twq = f_twe = nullptr;
+  d_tw = d_twh = d_twq = d_twe = d_t4 = nullptr;
+  f_tw = f_twh = f_twq = f_twe = f_t4 = nullptr;
   d_ptw = nullptr;
   f_ptw = nullptr;
 }

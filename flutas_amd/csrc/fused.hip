@@ -269,6 +269,19 @@ static LinePlan<T> plan_of(const LineTables &t) {
   return p;
 }
 
+// The tables a kind's tile kernels stage as their split and phase tables:
+// the DCT-IV/DST-IV pre/post twiddles for REDFT11/RODFT11.
+template <typename T>
+static void kind_tables(const LinePlan<T> &pl, int kind, const T **th, const T **tq) {
+  if (kind == K_REDFT11 || kind == K_RODFT11) {
+    *th = pl.t4;
+    *tq = pl.t4 + 2 * (pl.N + 1);
+  } else {
+    *th = pl.twh;
+    *tq = pl.twq;
+  }
+}
+
 // Per-pass twiddles of Plan<NC>: pass s >= 1, entry (rr, j) =
 // exp(-2 pi i rr j / (ns(s) r[s])), laid out [rr][j].
 template <int NC>
@@ -308,12 +321,14 @@ int launch_x_fwd(const LineTables &tx, int kind, const Tin *in, long long is1,
                  long long is2, T *Wt, WStride ws, int ny, int nz,
                  double *absmax_part, int *nf_flag, int nf_seq, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid(((ny + TJ - 1) / TJ) * nz);
     hipLaunchKernelGGL((k_x_fwd<T, Tin, NC, TJ, NT>), grid, dim3(NT), 0, s, in, is1,
-                       is2, Wt, ws, ny, kind, pl.tw, pl.twh, pl.twq, absmax_part, nf_flag,
+                       is2, Wt, ws, ny, kind, pl.tw, th, tq, absmax_part, nf_flag,
                        nf_seq);
   });
   FL_HIP(hipGetLastError());
@@ -325,16 +340,18 @@ int launch_x_bwd(const LineTables &tx, int kind, const T *Wt, WStride ws,
                  Tout *out, long long os1, long long os2, int ny, int nz,
                  T scale, int accumulate, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(tx);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   FL_DISPATCH(tx.n / 2, {
     constexpr int NT = tile_threads<T, NC>();
     constexpr int TJ = tile_lines<T, NC>();
     dim3 grid(((ny + TJ - 1) / TJ) * nz);
     if (accumulate)
       hipLaunchKernelGGL((k_x_bwd<T, Tout, NC, TJ, NT, true>), grid, dim3(NT), 0, s, Wt,
-                         ws, out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
+                         ws, out, os1, os2, ny, kind, scale, pl.tw, th, tq);
     else
       hipLaunchKernelGGL((k_x_bwd<T, Tout, NC, TJ, NT, false>), grid, dim3(NT), 0, s, Wt,
-                         ws, out, os1, os2, ny, kind, scale, pl.tw, pl.twh, pl.twq);
+                         ws, out, os1, os2, ny, kind, scale, pl.tw, th, tq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -385,18 +402,20 @@ int launch_yz_fwd(const LineTables &ty, int kind, T *Wt, WStride ws,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int pin, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
     constexpr int NTP = slab_threads<T, NC, true>();
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
-                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh,
-                         pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, th,
+                         tq);
     else
       hipLaunchKernelGGL((k_yz_fwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
-                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh,
-                         pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pin, pl.tw, th,
+                         tq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -408,16 +427,18 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
                   const double *lamy, const double *a, const double *b,
                   const double *c, int periodic, hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
     constexpr int NTP = slab_threads<T, NC, true>();
     constexpr int NTN = slab_threads<T, NC, false>();
     if (periodic)
       hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTP, true>), dim3(nx), dim3(NTP), 0, s, Wt, ws,
-                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, th, tq);
     else
       hipLaunchKernelGGL((k_zy_bwd<T, NC, GP, NTN, false>), dim3(nx), dim3(NTN), 0, s, Wt, ws,
-                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, pl.twh, pl.twq);
+                         ck, nx, nz, nzs, kind, lamx, lamy, a, b, c, pl.tw, th, tq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -463,6 +484,8 @@ int launch_yz_fwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZC
                      const double *a, const double *b, const double *c, int pin, int periodic,
                      hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   const dim3 grid(nx * zc.nchunk);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
@@ -471,10 +494,10 @@ int launch_yz_fwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZC
     if (zc.ngrp != (nz + GP - 1) / GP) return fail(FLUTAS_EINVAL, "z chunks: plane groups");
     if (periodic)
       hipLaunchKernelGGL((k_yz_fwd_zc<T, NC, GP, NTP, true>), grid, dim3(NTP), 0, s, Wt, ws, zc,
-                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, th, tq);
     else
       hipLaunchKernelGGL((k_yz_fwd_zc<T, NC, GP, NTN, false>), grid, dim3(NTN), 0, s, Wt, ws, zc,
-                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, th, tq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;
@@ -486,6 +509,8 @@ int launch_zy_bwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZC
                      const double *a, const double *b, const double *c, int pin, int periodic,
                      hipStream_t s) {
   const LinePlan<T> pl = plan_of<T>(ty);
+  const T *th, *tq;
+  kind_tables(pl, kind, &th, &tq);
   const dim3 grid(nx * zc.nchunk);
   FL_DISPATCH(ty.n / 2, {
     constexpr int GP = group_planes<T, NC>();
@@ -494,10 +519,10 @@ int launch_zy_bwd_zc(const LineTables &ty, int kind, T *Wt, WStride ws, const ZC
     if (zc.ngrp != (nz + GP - 1) / GP) return fail(FLUTAS_EINVAL, "z chunks: plane groups");
     if (periodic)
       hipLaunchKernelGGL((k_zy_bwd_zc<T, NC, GP, NTP, true>), grid, dim3(NTP), 0, s, Wt, ws, zc,
-                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, th, tq);
     else
       hipLaunchKernelGGL((k_zy_bwd_zc<T, NC, GP, NTN, false>), grid, dim3(NTN), 0, s, Wt, ws, zc,
-                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, pl.twh, pl.twq);
+                         nx, nz, msys, kind, lamx, lamy, a, b, c, pin, pl.tw, th, tq);
   });
   FL_HIP(hipGetLastError());
   return FLUTAS_OK;

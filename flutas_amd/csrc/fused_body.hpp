@@ -175,7 +175,10 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
   constexpr int LS = line_stride<T, n>();
   constexpr int LD = (TJ * n + NT - 1) / NT;
   const int nl = min(TJ, ny - j0);
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   Tin r[LD];
   // branch-free loads: lines past ny (partial tile) re-read line ny-1
 #pragma unroll
@@ -191,8 +194,8 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
     const int l = t / n, e = t - (t / n) * n;
     T x = (T)r[q];
     int pos = e;
-    if (trig) {
-      pos = tile::mk_pos(e, n);
+    if (trig || d4) {
+      pos = trig ? tile::mk_pos(e, n) : tile::d4_pos(e, n);
       if (sine && (e & 1)) x = -x;
     }
     sm[l * LS + pos] = x;
@@ -214,8 +217,13 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
     if (bad) *nf_flag = nf_seq;
   }
   __syncthreads();
+  if (d4) {
+    tile::d4_pre<T, NC, TJ, NT, LS, false>(sm, W.th);
+    __syncthreads();
+  }
   tile::cfft<T, -1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
-  tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  if (d4) tile::d4_post<T, NC, TJ, NT, LS, false>(sm, W.tq, sine);
+  else tile::post_fwd<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
   // transposed store
   T *plane = Wt + (long long)k * ws.wk;
 #pragma unroll
@@ -240,7 +248,10 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
   constexpr int LS = line_stride<T, n>();
   constexpr int LD = (TJ * n + NT - 1) / NT;
   const int nl = min(TJ, ny - j0);
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   const T *plane = Wt + (long long)k * ws.wk;
   T r[LD];
 #pragma unroll
@@ -263,13 +274,28 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int t = threadIdx.x + q * NT;
-    if (t < TJ * n) sm[(t % TJ) * LS + t / TJ] = r[q];
+    if (t < TJ * n) {
+      const int i = t / TJ;
+      T x = r[q];
+      int pos = i;
+      if (d4) {
+        pos = tile::d4_pos(i, n);
+        if (sine && (i & 1)) x = -x;
+      }
+      sm[(t % TJ) * LS + pos] = x;
+    }
   }
   const Tw<T, NC> W =
       stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw);
   __syncthreads();
-  tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  if (d4) {
+    tile::d4_pre<T, NC, TJ, NT, LS, true>(sm, W.th);
+    __syncthreads();
+  } else {
+    tile::pre_inv<T, NC, TJ, NT, LS>(sm, kind, W.th, W.tq);
+  }
   tile::cfft<T, +1, NC, TJ, NT, LS / 2>(reinterpret_cast<cx<T> *>(sm), W.tw);
+  if (d4) tile::d4_post<T, NC, TJ, NT, LS, true>(sm, W.tq, sine);
 #pragma unroll
   for (int q = 0; q < LD; ++q) {
     const int t = threadIdx.x + q * NT;
@@ -401,7 +427,10 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
   const int nck = checkpoints<GP>(nz);
   const long long pstride = ws.wk;       // plane stride
   T *line0 = Wt + ws.ioff(i);            // plane 0 line of slab i
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   const double lx = lamx[i];
   double lam[CPT], dprev[CPT], pprev[CPT];
   // periodic z only: second right-hand side, sums giving p1(0) and p2(0)
@@ -460,18 +489,24 @@ __device__ __forceinline__ void yz_fwd_slab(T *__restrict__ Wt, WStride ws,
       const int t = threadIdx.x + q * NT;
       if ((GP * n / 2) % NT != 0 && t >= GP * n / 2) continue;
       const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
-      const int p0 = trig ? (e >> 1) : e, p1 = trig ? n - 1 - (e >> 1) : e + 1;
+      const int p0 = trig ? (e >> 1) : e;
+      const int p1 = trig ? n - 1 - (e >> 1) : (d4 ? n - 1 - e : e + 1);
       A[l * LS + p0] = r[q].x;
-      A[l * LS + p1] = (trig && sine) ? -r[q].y : r[q].y;
+      A[l * LS + p1] = ((trig || d4) && sine) ? -r[q].y : r[q].y;
     }
     __syncthreads();
     if (g > 0) flush(g - 1);
     // the last group re-reads itself (the values are not used), so the
     // loop carries no branch around the prefetch
     load(min(g + 1, ngrp - 1));
+    if (d4) {
+      tile::d4_pre<T, NC, GP, NT, LS, false>(A, W.th);
+      __syncthreads();
+    }
     tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
                                             reinterpret_cast<cx<T> *>(B), W.tw);
-    tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
+    if (d4) tile::d4_post_pp<T, NC, GP, NT, LS, false>(Z, Y, W.tq, sine);
+    else tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
     // forward elimination, plane by plane (dgtsv_homebrewed order)
 #pragma unroll
     for (int l = 0; l < GP; ++l) {
@@ -586,7 +621,10 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
   const int nck = checkpoints<GP>(nz);
   const long long pstride = ws.wk;
   T *line0 = Wt + ws.ioff(i);
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   const double lx = lamx[i];
   double lam[CPT], pnext[CPT];
   // periodic z only: p2 of the next plane, the closed last plane
@@ -735,15 +773,18 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
     for (int u = 0; u < CPT; ++u) {
       const int j = CPT * threadIdx.x + u;
       if (CPT * NT != n && j >= n) continue;
+      const int jp = d4 ? tile::d4_pos(j, n) : j;
+      const bool neg = d4 && sine && (j & 1);
 #pragma unroll
-      for (int l = 0; l < GP; ++l)
-        A[l * LS + j] = pv[l][u];  // lines past np are not stored
+      for (int l = 0; l < GP; ++l) A[l * LS + jp] = neg ? -pv[l][u] : pv[l][u];
     }
     __syncthreads();
     if (g > 0) load(g - 1);
-    tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
+    if (d4) tile::d4_pre_pp<T, NC, GP, NT, LS, true>(A, B, W.th);
+    else tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
     tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
                                             reinterpret_cast<cx<T> *>(A), W.tw);
+    if (d4) tile::d4_post<T, NC, GP, NT, LS, true>(R, W.tq, sine);
   }
   flush(0);
 }

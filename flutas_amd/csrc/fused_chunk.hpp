@@ -95,7 +95,10 @@ __device__ __forceinline__ void yz_fwd_chunk(T *__restrict__ Wt, WStride ws, con
   const int gb = ks / GP, ge = (ke + GP - 1) / GP;
   const long long pstride = ws.wk;
   T *line0 = Wt + ws.ioff(i);
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   const double lx = lamx[i];
   double lam[CPT], dprev[CPT], pprev[CPT], w[CPT], S[CPT], E[CPT], ym[CPT];
   // d of the plane before the chunk (not used by the first chunk)
@@ -139,16 +142,22 @@ __device__ __forceinline__ void yz_fwd_chunk(T *__restrict__ Wt, WStride ws, con
       const int t = threadIdx.x + q * NT;
       if ((GP * n / 2) % NT != 0 && t >= GP * n / 2) continue;
       const int l = t / (n / 2), e = 2 * (t - l * (n / 2));
-      const int p0 = trig ? (e >> 1) : e, p1 = trig ? n - 1 - (e >> 1) : e + 1;
+      const int p0 = trig ? (e >> 1) : e;
+      const int p1 = trig ? n - 1 - (e >> 1) : (d4 ? n - 1 - e : e + 1);
       A[l * LS + p0] = r[q].x;
-      A[l * LS + p1] = (trig && sine) ? -r[q].y : r[q].y;
+      A[l * LS + p1] = ((trig || d4) && sine) ? -r[q].y : r[q].y;
     }
     __syncthreads();
     if (g > gb) flush(g - 1);
     load(min(g + 1, ge - 1));
+    if (d4) {
+      tile::d4_pre<T, NC, GP, NT, LS, false>(A, W.th);
+      __syncthreads();
+    }
     tile::cfft_pp<T, -1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(A),
                                             reinterpret_cast<cx<T> *>(B), W.tw);
-    tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
+    if (d4) tile::d4_post_pp<T, NC, GP, NT, LS, false>(Z, Y, W.tq, sine);
+    else tile::post_fwd_pp<T, NC, GP, NT, LS>(Z, Y, kind, W.th, W.tq);
 #pragma unroll
     for (int l = 0; l < GP; ++l) {
       if (l >= np) {
@@ -231,7 +240,10 @@ __device__ __forceinline__ void zy_bwd_chunk(T *__restrict__ Wt, WStride ws, con
   const int gb = ks / GP, ge = (ke + GP - 1) / GP;
   const long long pstride = ws.wk;
   T *line0 = Wt + ws.ioff(i);
-  const bool trig = tile::is_trig(kind), sine = tile::is_sine(kind);
+  // d4: REDFT11 / RODFT11 (DCT-IV / DST-IV, tile_fft.hpp)
+  const bool d4 = tile::is_d4(kind);
+  const bool trig = tile::is_trig(kind) && !d4;
+  const bool sine = tile::is_sine(kind) || kind == K_RODFT11;
   const double lx = lamx[i];
   const int nch = zc.nchunk;
   double lam[CPT];
@@ -412,14 +424,18 @@ __device__ __forceinline__ void zy_bwd_chunk(T *__restrict__ Wt, WStride ws, con
     for (int u = 0; u < CPT; ++u) {
       const int j = CPT * threadIdx.x + u;
       if (CPT * NT != n && j >= n) continue;
+      const int jp = d4 ? tile::d4_pos(j, n) : j;
+      const bool neg = d4 && sine && (j & 1);
 #pragma unroll
-      for (int l = 0; l < GP; ++l) A[l * LS + j] = pv[l][u];
+      for (int l = 0; l < GP; ++l) A[l * LS + jp] = neg ? -pv[l][u] : pv[l][u];
     }
     __syncthreads();
     if (g > gb) load(g - 1);
-    tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
+    if (d4) tile::d4_pre_pp<T, NC, GP, NT, LS, true>(A, B, W.th);
+    else tile::pre_inv_pp<T, NC, GP, NT, LS>(A, B, kind, W.th, W.tq);
     tile::cfft_pp<T, +1, NC, GP, NT, LS / 2>(reinterpret_cast<cx<T> *>(B),
                                             reinterpret_cast<cx<T> *>(A), W.tw);
+    if (d4) tile::d4_post<T, NC, GP, NT, LS, true>(R, W.tq, sine);
   }
   flush(gb);
 }

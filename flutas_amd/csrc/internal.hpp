@@ -61,6 +61,9 @@ struct LinePlan {
   const T *twh;  // 2(N+1):  exp(-2 pi i k / n), k <= N
   const T *twq;  // 2(n+1):  exp(-i pi k / (2n)), k <= n
   const T *twe;  // 2N:      exp(-i pi (4m+1) / (4n)), m < N (DCT-IV/DST-IV)
+  // 4(N+1): exp(-i pi (4m+1) / (4n)), m <= N, then exp(-i pi k / n), k <= N
+  // (the DCT-IV/DST-IV tables of the tile kernels, sized as twh)
+  const T *t4;
 };
 
 // Host-owned device tables for one length, in double and float.
@@ -69,6 +72,8 @@ struct LineTables {
   int radix[kMaxStages] = {};
   double *d_tw = nullptr, *d_twh = nullptr, *d_twq = nullptr, *d_twe = nullptr;
   float *f_tw = nullptr, *f_twh = nullptr, *f_twq = nullptr, *f_twe = nullptr;
+  double *d_t4 = nullptr;
+  float *f_t4 = nullptr;
   // per-pass twiddle tables of the fused kernels (set by
   // build_fused_twiddles when the length has a compiled plan)
   double *d_ptw = nullptr;

@@ -127,7 +127,7 @@ static int find_fft(char b0, char b1, char cf, int &kf, int &kb, double &norm0) 
 
 // forward r2r kinds whose tile transform (post_fwd) the fused P1/P2 run
 static bool fused_forward_kind(int k) {
-  return k == K_R2HC || k == K_REDFT10 || k == K_RODFT10;
+  return k == K_R2HC || k == K_REDFT10 || k == K_RODFT10 || k == K_REDFT11 || k == K_RODFT11;
 }
 
 }  // namespace flutas
@@ -1501,10 +1501,23 @@ int flutas_batch_create_ex(int nprob, const int *ng, const char *cbc, const doub
     p.W = precision == FLUTAS_FP64 ? (void *)h->W64 : (void *)h->W32;
     p.R = h->R32;
     p.lamx = h->d_lamx; p.lamy = h->d_lamy; p.a = h->d_a; p.b = h->d_b; p.c = h->d_c;
-    p.twx = h->tx.d_ptw; p.twhx = h->tx.d_twh; p.twqx = h->tx.d_twq;
-    p.twy = h->ty.d_ptw; p.twhy = h->ty.d_twh; p.twqy = h->ty.d_twq;
-    p.ftwx = h->tx.f_ptw; p.ftwhx = h->tx.f_twh; p.ftwqx = h->tx.f_twq;
-    p.ftwy = h->ty.f_ptw; p.ftwhy = h->ty.f_twh; p.ftwqy = h->ty.f_twq;
+    // split/phase tables, or the DCT-IV/DST-IV pre/post tables of the ND
+    // and DN kinds (REDFT11/RODFT11)
+    const bool d4x = (p.kx_f == K_REDFT11 || p.kx_f == K_RODFT11);
+    const bool d4y = (p.ky_f == K_REDFT11 || p.ky_f == K_RODFT11);
+    const size_t ox = 2 * (size_t)(h->tx.N + 1), oy = 2 * (size_t)(h->ty.N + 1);
+    p.twx = h->tx.d_ptw;
+    p.twhx = d4x ? h->tx.d_t4 : h->tx.d_twh;
+    p.twqx = d4x ? h->tx.d_t4 + ox : h->tx.d_twq;
+    p.twy = h->ty.d_ptw;
+    p.twhy = d4y ? h->ty.d_t4 : h->ty.d_twh;
+    p.twqy = d4y ? h->ty.d_t4 + oy : h->ty.d_twq;
+    p.ftwx = h->tx.f_ptw;
+    p.ftwhx = d4x ? h->tx.f_t4 : h->tx.f_twh;
+    p.ftwqx = d4x ? h->tx.f_t4 + ox : h->tx.f_twq;
+    p.ftwy = h->ty.f_ptw;
+    p.ftwhy = d4y ? h->ty.f_t4 : h->ty.f_twh;
+    p.ftwqy = d4y ? h->ty.f_t4 + oy : h->ty.f_twq;
     // checkpoints for the quarter plane group (the most groups)
     const int gp = batch_group_planes(h->ny, b->elem, 2);
     const size_t nck = (size_t)((h->nz + gp - 1) / gp) * (p.per ? 2 : 1);

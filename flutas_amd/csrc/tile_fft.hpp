@@ -292,6 +292,115 @@ __device__ __forceinline__ int mk_pos(int e, int n) {
   return (e & 1) ? (n - 1 - (e >> 1)) : (e >> 1);
 }
 
+// ---- DCT-IV / DST-IV (REDFT11 / RODFT11: the ND and DN pairs) ----------
+// Y_k = 2 sum_j x_j cos(pi (j + 1/2)(k + 1/2) / n) through a complex FFT of
+// length N = n/2: v_m = x_2m + i x_{n-1-2m}, w_m = v_m exp(-i pi (4m+1)/4n),
+// W = FFT_-(w), u_k = W_k exp(-i pi k/n), Y_2k = 2 Re u_k,
+// Y_{n-1-2k} = -2 Im u_k.  DST-IV: the same on (-1)^j x_j, output
+// reversed.  Each is its own inverse up to 2n, so the backward passes run
+// the same transform; their FFT has the + sign, which gives FFT_- through
+// FFT_-(w) = conj(FFT_+(conj w)) (CJ below).  The pre and post tables
+// (exp(-i pi (4m+1)/4n), exp(-i pi k/n)) take the staged slots of the split
+// and phase tables, which these kinds do not use.
+__device__ __forceinline__ bool is_d4(int kind) { return kind == K_REDFT11 || kind == K_RODFT11; }
+// position of element e in the packed line (Re v_m at 2m, Im v_m at 2m+1)
+__device__ __forceinline__ int d4_pos(int e, int n) { return (e & 1) ? (n - e) : e; }
+
+// w_m = v_m pre_m (conjugated when CJ), element-wise in place: every thread
+// rewrites the values it reads, so no barrier inside.
+template <typename T, int N, int L, int NT, int LS, bool CJ>
+__device__ __forceinline__ void d4_pre(T *sm, const T *__restrict__ pre) {
+  constexpr int TOT = L * N;
+  constexpr int PER = (TOT + NT - 1) / NT;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
+    const int l = t / N, m = t - (t / N) * N;
+    cx<T> *Z = reinterpret_cast<cx<T> *>(sm + l * LS);
+    const cx<T> v = mul(Z[m], cx<T>{pre[2 * m], pre[2 * m + 1]});
+    Z[m] = CJ ? cj(v) : v;
+  }
+}
+// the same from src into dst, then a barrier
+template <typename T, int N, int L, int NT, int LS, bool CJ>
+__device__ __forceinline__ void d4_pre_pp(const T *__restrict__ src, T *__restrict__ dst,
+                                          const T *__restrict__ pre) {
+  constexpr int TOT = L * N;
+  constexpr int PER = (TOT + NT - 1) / NT;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
+    const int l = t / N, m = t - (t / N) * N;
+    const cx<T> *Z = reinterpret_cast<const cx<T> *>(src + l * LS);
+    const cx<T> v = mul(Z[m], cx<T>{pre[2 * m], pre[2 * m + 1]});
+    reinterpret_cast<cx<T> *>(dst + l * LS)[m] = CJ ? cj(v) : v;
+  }
+  __syncthreads();
+}
+// Output of slot k of the FFT (conjugated when CJ): the two real values
+// and their positions (reversed for the sine kind).
+template <typename T, int N, bool CJ>
+__device__ __forceinline__ void d4_out(cx<T> W, int k, const T *__restrict__ post, bool sine,
+                                       T &v0, int &p0, T &v1, int &p1) {
+  constexpr int n = 2 * N;
+  const cx<T> u = mul(CJ ? cj(W) : W, cx<T>{post[2 * k], post[2 * k + 1]});
+  v0 = T(2) * u.x;
+  v1 = T(-2) * u.y;
+  p0 = sine ? n - 1 - 2 * k : 2 * k;
+  p1 = sine ? 2 * k : n - 1 - 2 * k;
+}
+// in place, register staged (barrier between the reads and the writes, and
+// after the writes)
+template <typename T, int N, int L, int NT, int LS, bool CJ>
+__device__ __forceinline__ void d4_post(T *sm, const T *__restrict__ post, bool sine) {
+  constexpr int TOT = L * N;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  T o[PER][2];
+  int ps[PER][2];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    ps[u][0] = ps[u][1] = -1;
+    if (TOT % NT != 0 && t >= TOT) continue;
+    const int l = t / N, k = t - (t / N) * N;
+    const cx<T> W = reinterpret_cast<const cx<T> *>(sm + l * LS)[k];
+    d4_out<T, N, CJ>(W, k, post, sine, o[u][0], ps[u][0], o[u][1], ps[u][1]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (ps[u][0] < 0) continue;
+    T *line = sm + (t / N) * LS;
+    line[ps[u][0]] = o[u][0];
+    line[ps[u][1]] = o[u][1];
+  }
+  __syncthreads();
+}
+// from src into dst, then a barrier
+template <typename T, int N, int L, int NT, int LS, bool CJ>
+__device__ __forceinline__ void d4_post_pp(const T *__restrict__ src, T *__restrict__ dst,
+                                           const T *__restrict__ post, bool sine) {
+  constexpr int TOT = L * N;
+  constexpr int PER = (TOT + NT - 1) / NT;
+#pragma unroll 2
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * NT;
+    if (TOT % NT != 0 && t >= TOT) continue;
+    const int l = t / N, k = t - (t / N) * N;
+    const cx<T> W = reinterpret_cast<const cx<T> *>(src + l * LS)[k];
+    T v0, v1;
+    int p0, p1;
+    d4_out<T, N, CJ>(W, k, post, sine, v0, p0, v1, p1);
+    T *line = dst + l * LS;
+    line[p0] = v0;
+    line[p1] = v1;
+  }
+  __syncthreads();
+}
+
 // Real split of the pair (k, N-k), 0 <= k <= N/2, of the complex FFT Z of
 // a packed real line (Z_N := Z_0): with A = Z_k + conj Z_{N-k},
 // B = Z_k - conj Z_{N-k} and w = exp(-2 pi i k/n),

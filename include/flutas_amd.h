@@ -74,8 +74,8 @@ int flutas_pointer_is_device(const void *ptr, int *is_device);
  * src/initsolver.f90:164-178 belong to other transforms) and 'f' with
  * FLUTAS_FP32_REFINE (its residual is the cell-centred operator); odd ng
  * (EINVAL, src/sanity.f90:155-158).  ND/DN pairs in x or y use REDFT11 /
- * RODFT11 ('c') or RODFT01 forward ('f' DN) on the five-pass line
- * pipeline. */
+ * RODFT11 ('c'; DCT-IV / DST-IV tiles on the four-pass pipeline at the
+ * compiled lengths) or RODFT01 forward ('f' DN, five-pass line pipeline). */
 int flutas_psolve_create(const int ng[3], const char cbc[6], const double bc[6],
                          const char c_or_f[3], const double dli[3], int nh_d,
                          const double *dzci, const double *dzfi, int precision,

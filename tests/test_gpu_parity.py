@@ -112,6 +112,16 @@ SOLVE_CASES = [
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
     ((20, 36, 12), (1.0, 1.0, 1.0), "DNNDPP", 0.0),
+    # the same kinds on the fused pipeline (DCT-IV / DST-IV tiles): x and y,
+    # periodic z, radix-5 lengths, 512-point y-lines in the blocked field,
+    # and z chunks (64 x-slabs, 100 planes)
+    ((64, 32, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
+    ((32, 64, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),
+    ((64, 48, 24), (1.0, 1.0, 1.0), "DNNDPP", 0.0),
+    ((80, 160, 12), (1.0, 1.0, 1.0), "NDDNNN", 0.0),
+    ((32, 512, 12), (1.0, 1.0, 1.0), "PPNDNN", 1.5),
+    ((512, 32, 10), (1.0, 1.0, 1.0), "DNPPDD", 0.0),
+    ((64, 64, 100), (1.0, 1.0, 1.0), "NDPPNN", 1.5),
 ]
 
 
@@ -355,7 +365,7 @@ def test_projection_step_channel_64(api, oracle, alpha):
 
 
 @pytest.mark.parametrize("cbc,alpha", [("PPPPNN", 1.5), ("NNNNNN", 0.0), ("PPPPPP", 0.0),
-                                       ("PPPPDD", 1.3)])
+                                       ("PPPPDD", 1.3), ("NDDNNN", 1.2)])
 def test_fp32_refined_reaches_tolerance(api, oracle, cbc, alpha):
     ng = (64, 512, 40) if cbc == "PPPPDD" else (64, 48, 40)
     lengths = (2 * np.pi, np.pi, 1.0)

@@ -23,6 +23,9 @@ PROBLEMS = [
     ((48, 192, 34), (1.0, 1.0, 2.0), "NNPPPP", 0.0),
     ((96, 256, 4), (1.0, 1.0, 1.0), "DDNNNN", 0.0),
     ((192, 128, 70), (1.0, 1.0, 1.0), "PPPPNN", 1.8),
+    # ND / DN pairs (DCT-IV / DST-IV tiles)
+    ((64, 48, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
+    ((32, 64, 16), (1.0, 1.0, 1.0), "PPDNDD", 0.0),
 ]
 
 

@@ -670,7 +670,9 @@ struct Plane7 {
   double f, c, xm, xp, ym, yp;
 };
 
-template <typename Tr>
+// STORE: r receives the residual (held in registers through the walk);
+// without it only the maximum is formed, with no registers held.
+template <typename Tr, bool STORE>
 __global__ void __launch_bounds__(256)
     k_residual(OperatorDesc op, const double *__restrict__ f,
                const double *__restrict__ p, Tr *__restrict__ r,
@@ -719,7 +721,7 @@ __global__ void __launch_bounds__(256)
     // stored after the walk, so no store shares the load counter inside it
     Plane7 cur = gather(k0);
     Plane7 nxt = gather(knext(k0));
-    Tr hold[kResidualPlanes];
+    Tr hold[STORE ? kResidualPlanes : 1];
 #pragma unroll
     for (int q = 0; q < kResidualPlanes; ++q) {
       const int k = k0 + q;
@@ -736,14 +738,14 @@ __global__ void __launch_bounds__(256)
       if (k > 0 || zper) lz += op.a[k] * pzm;
       if (k < nz - 1 || zper) lz += op.c[k] * pzp;
       const double res = cur.f - ((lx + ly) + lz);
-      hold[q] = (Tr)res;
+      if constexpr (STORE) hold[q] = (Tr)res;
       const double ar = fabs(res);
       if (ar > am || res != res) am = (res != res) ? res : ar;
       pzm = pc;
       cur = nxt;
       nxt = nxt2;
     }
-    if (r) {
+    if constexpr (STORE) {
 #pragma unroll
       for (int q = 0; q < kResidualPlanes; ++q)
         if (k0 + q < k1) r[i + (long long)nx * (j + (long long)ny * (k0 + q))] = hold[q];
@@ -868,7 +870,12 @@ int launch_residual(const OperatorDesc &op, const double *f, const double *p,
   const int nbx = (op.nx + bx - 1) / bx;
   const long long nb = (long long)nbx * op.ny * ((op.nz + kResidualPlanes - 1) / kResidualPlanes);
   if (nb > 0x7fffffffLL) return fail(FLUTAS_EINVAL, "residual grid too large");
-  hipLaunchKernelGGL(k_residual<Tr>, dim3((unsigned)nb), dim3(bx), 0, s, op, f, p, r, partial, nbx);
+  if (r)
+    hipLaunchKernelGGL((k_residual<Tr, true>), dim3((unsigned)nb), dim3(bx), 0, s, op, f, p, r,
+                       partial, nbx);
+  else
+    hipLaunchKernelGGL((k_residual<Tr, false>), dim3((unsigned)nb), dim3(bx), 0, s, op, f, p, r,
+                       partial, nbx);
   FL_HIP(hipGetLastError());
   *nparts = (int)nb;
   return FLUTAS_OK;

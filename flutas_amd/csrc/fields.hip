@@ -717,11 +717,11 @@ __global__ void __launch_bounds__(256)
     auto knext = [&](int k) { return (k + 1 < nz) ? k + 1 : (zper ? k + 1 - nz : nz - 1); };
     double pzm = (k0 > 0) ? p[c0 + sz * k0] : (zper ? p[c0 + sz * nz] : 0.0);
     // two planes in flight: plane k+1 is loaded while plane k is evaluated
-    // and plane k+2 is requested; the residuals are held in registers and
-    // stored after the walk, so no store shares the load counter inside it
+    // and plane k+2 is requested; plane k's residual is stored after plane
+    // k+2's loads are issued (the shared load/store counter completes in
+    // issue order, so no later wait for loads waits for it)
     Plane7 cur = gather(k0);
     Plane7 nxt = gather(knext(k0));
-    Tr hold[STORE ? kResidualPlanes : 1];
 #pragma unroll
     for (int q = 0; q < kResidualPlanes; ++q) {
       const int k = k0 + q;
@@ -738,17 +738,14 @@ __global__ void __launch_bounds__(256)
       if (k > 0 || zper) lz += op.a[k] * pzm;
       if (k < nz - 1 || zper) lz += op.c[k] * pzp;
       const double res = cur.f - ((lx + ly) + lz);
-      if constexpr (STORE) hold[q] = (Tr)res;
+      // the store follows the loads of plane k+2 in issue order, so the
+      // waits for later planes' loads do not wait for it
+      if constexpr (STORE) r[i + (long long)nx * (j + (long long)ny * k)] = (Tr)res;
       const double ar = fabs(res);
       if (ar > am || res != res) am = (res != res) ? res : ar;
       pzm = pc;
       cur = nxt;
       nxt = nxt2;
-    }
-    if constexpr (STORE) {
-#pragma unroll
-      for (int q = 0; q < kResidualPlanes; ++q)
-        if (k0 + q < k1) r[i + (long long)nx * (j + (long long)ny * (k0 + q))] = hold[q];
     }
   }
   smax[threadIdx.x] = am;

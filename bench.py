@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the pressure Poisson solve on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--line-path]
 
 A step is one solve of the 512^3 stretched-z channel (periodic x and y,
 Neumann z walls, Lx = 2 pi, Ly = pi, tanh stretching 1.5) on a seeded N(0,1)
@@ -262,7 +262,15 @@ def main():
                     help="skip the projection step around the solve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run the plain copy the byte-counter summaries calibrate on")
+    ap.add_argument("--line-path", action="store_true",
+                    help="run the five-pass line pipeline (separate x, y, z, y, x launches; "
+                         "FLUTAS_UNFUSED=1) instead of the four fused passes")
     args = ap.parse_args()
+    if args.line_path:
+        # read by the library when a plan is created; the batched solves
+        # run on the fused passes only
+        os.environ["FLUTAS_UNFUSED"] = "1"
+        args.no_batch = True
     if args.gpus != 1:
         print("bench.py measures one GPU only; run it with --gpus 1 "
               "(the multi-GPU slab exchange is a design, not built).", file=sys.stderr)
@@ -294,7 +302,8 @@ def main():
     stages = plan.profile(f, p, args.profile_reps)
     stage_rows = []
     for s_idx, (name, t_between) in enumerate(stages):
-        t = plan.time_stage(f, p, s_idx, reps=8)
+        # the line pipeline's stages are timed by the events between them
+        t = t_between if args.line_path else plan.time_stage(f, p, s_idx, reps=8)
         b = STAGE_BYTES_FP64 * npts
         stage_rows.append({"stage": name, "ms": t, "GBps": b / (t * 1e6),
                            "ms_events_between_stages": t_between})
@@ -353,11 +362,14 @@ def main():
                                f"BC {case.cbc}, Lx,Ly,Lz={tuple(round(v, 6) for v in case.lengths)}, "
                                f"tanh z stretching {case.alpha}, seeded N(0,1) RHS "
                                "(dz-weighted mean removed), inputs resident in HBM",
-                   "ng": list(case.ng), "bc": case.cbc},
+                   "ng": list(case.ng), "bc": case.cbc,
+                   "pipeline": "five-pass line path" if args.line_path else "four fused passes"},
         "roofline": {"bound": "hbm", "kernel": dom["stage"],
-                     "time_source": "HIP events on the library stream around 8 back-to-back "
-                                    "launches of the kernel after a solve "
-                                    "(flutas_psolve_time_stage)",
+                     "time_source": ("HIP events between the stages of whole solves "
+                                     "(flutas_psolve_profile)" if args.line_path else
+                                     "HIP events on the library stream around 8 back-to-back "
+                                     "launches of the kernel after a solve "
+                                     "(flutas_psolve_time_stage)"),
                      "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": STAGE_BYTES_FP64 * npts,

@@ -642,16 +642,18 @@ __device__ __forceinline__ void zy_bwd_slab(T *__restrict__ Wt, WStride ws,
   double dck[CPT], q2ck[CPT];
   auto load = [&](int g) {
     const int k0 = g * GP;
-    // branch-free: out-of-range planes/columns re-read valid entries
-#pragma unroll
-    for (int l = 0; l < GP; ++l)
-      load_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, nz - 1) * pstride, pv[l], n);
     // the recurrence restarts at the end of group g - 1 (unused, but read
-    // in bounds, for g = 0)
+    // in bounds, for g = 0); the checkpoint is issued first, so that the
+    // factors can be recomputed while the lines are still arriving (loads
+    // complete in issue order)
     const int c0 = max(g - 1, 0);
     const double *cg = ck + ((long long)i * nck + c0) * n;
     load_cols<double, CPT, NT>(cg, dck, n);
     if constexpr (PER) load_cols<double, CPT, NT>(cg + (long long)nx * nck * n, q2ck, n);
+    // branch-free: out-of-range planes/columns re-read valid entries
+#pragma unroll
+    for (int l = 0; l < GP; ++l)
+      load_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, nz - 1) * pstride, pv[l], n);
   };
   // Stores of group g's lines (element pairs, 16-byte stores in FP64) from
   // R.  They are issued in the next group's step, after its loads have

@@ -314,13 +314,14 @@ __device__ __forceinline__ void zy_bwd_chunk(T *__restrict__ Wt, WStride ws, con
   double dck[CPT], mck[CPT], qck[CPT];
   auto load = [&](int g) {
     const int k0 = g * GP;
-#pragma unroll
-    for (int l = 0; l < GP; ++l)
-      load_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, ke - 1) * pstride, pv[l], n);
+    // the group's starting values first (see zy_bwd_slab), then its lines
     const long long o = ((long long)i * zc.ngrp + g) * n;
     load_cols<double, CPT, NT>(zc.ckd + o, dck, n);
     load_cols<double, CPT, NT>(zc.ckm + o, mck, n);
     if constexpr (PER) load_cols<double, CPT, NT>(zc.ckq + o, qck, n);
+#pragma unroll
+    for (int l = 0; l < GP; ++l)
+      load_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, ke - 1) * pstride, pv[l], n);
   };
   constexpr int LD2 = (GP * n / 2 + NT - 1) / NT;
   auto flush = [&](int g) {

@@ -131,7 +131,7 @@ __device__ __forceinline__ void yz_fwd_chunk(T *__restrict__ Wt, WStride ws, con
     const int k0 = g * GP;
 #pragma unroll
     for (int l = 0; l < GP; ++l)
-      store_cols<T, CPT, NT>(line0 + (long long)min(k0 + l, ke - 1) * pstride, hold[l], n);
+      store_cols_nt<T, CPT, NT>(line0 + (long long)min(k0 + l, ke - 1) * pstride, hold[l], n);
   };
   load(gb);
   for (int g = gb; g < ge; ++g) {
@@ -339,7 +339,7 @@ __device__ __forceinline__ void zy_bwd_chunk(T *__restrict__ Wt, WStride ws, con
       } else {
         v = *reinterpret_cast<const V2 *>(R + l * LS + e);
       }
-      *reinterpret_cast<V2 *>(line0 + (long long)(k0 + l) * pstride + e) = v;
+      st_nt2(v.x, v.y, line0 + (long long)(k0 + l) * pstride + e);
     }
   };
   load(ge - 1);

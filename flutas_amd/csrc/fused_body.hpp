@@ -157,15 +157,15 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 // those run one workgroup (two waves per SIMD) per CU.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
-// Threads of a P1/P4 tile workgroup: 512 for lines of 512 points and for
-// FP64 lines of 1024 points (measured faster than 256: P4 0.49 -> 0.45 ms
-// at 512^3; P1 3.29 -> 2.70 ms and P4 3.91 -> 3.35 ms at 1024x1024x512),
-// 256 otherwise (faster for the short lines of batched problems; the FP32
-// accumulating P4 of 1024-point lines would spill at 128 registers).
+// Threads of a P1/P4 tile workgroup: 512 for lines of 512 and 1024 points
+// (measured faster than 256: P4 0.49 -> 0.45 ms at 512^3; P1 3.29 ->
+// 2.70 ms and P4 3.91 -> 3.35 ms at 1024x1024x512 in FP64, the FP32 P1
+// 1.59 -> 1.34 ms), 256 otherwise (faster for the short lines of batched
+// problems).
 template <typename T, int NC>
 constexpr int tile_threads() {
   if (2 * NC == 512) return 512;
-  if (2 * NC == 1024 && sizeof(T) == 8) return 512;
+  if (2 * NC == 1024) return 512;
   return 256;
 }
 
@@ -284,8 +284,13 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
   }
   // the previous values of an accumulated output are loaded with the tile
   // so that their latency overlaps the transform
-  Tout prev[ACC ? LD : 1];
-  if constexpr (ACC) {
+  // FP32 1024-point tiles load the accumulated values at the store: 64 of
+  // them per lane would not fit beside the tile's registers at 512 threads
+  // (refined 1024x1024x512: this pass 3.34 -> 2.63 ms with 512 threads
+  // against the preloading 256-thread form)
+  constexpr bool LATE = ACC && NC == 512 && sizeof(T) == 4;
+  Tout prev[(ACC && !LATE) ? LD : 1];
+  if constexpr (ACC && !LATE) {
 #pragma unroll
     for (int q = 0; q < LD; ++q) {
       const int t = min((int)threadIdx.x + q * NT, TJ * n - 1);
@@ -331,7 +336,8 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
     }
     const T v = sm[l * LS + pos] * (scale * sg);
     Tout *o = out + (long long)(j0 + l) * os1 + (long long)k * os2 + e;
-    if constexpr (ACC) *o = (Tout)((double)prev[q] + (double)v);
+    if constexpr (LATE) *o = (Tout)((double)*o + (double)v);
+    else if constexpr (ACC) *o = (Tout)((double)prev[q] + (double)v);
     else if constexpr (sizeof(T) == 8) st_nt((Tout)v, o);
     else *o = (Tout)v;
   }

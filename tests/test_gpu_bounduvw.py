@@ -1,7 +1,8 @@
 """bounduvw on the GPU (src/bound.f90:17-144, set_bc :227-646, outflow
 :649-773) against the oracle, bit for bit: every halo value of u, v, w,
 edges and corners included, for periodic, wall and outflow patterns and
-halo widths 1 to 3."""
+halo widths 1 to 3, in one launch (every n > 2 nh) and as the per-face
+launches (small grids)."""
 import numpy as np
 import pytest
 
@@ -42,9 +43,10 @@ def _pattern(name, rng):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["channel", "cavity", "inflow_outflow", "all_outflow"])
-@pytest.mark.parametrize("nh_u", [1, 2, 3])
-def test_bounduvw_matches_oracle(api, oracle, name, nh_u):
-    ng = (12, 10, 16)
+@pytest.mark.parametrize("nh_u,ng", [(1, (12, 10, 16)), (2, (12, 10, 16)), (3, (12, 10, 16)),
+                                     # some n <= 2 nh: the per-face launches
+                                     (3, (4, 6, 5)), (2, (6, 4, 3))])
+def test_bounduvw_matches_oracle(api, oracle, name, nh_u, ng):
     nh_d = nh_u
     rng = np.random.default_rng(nh_u * 7 + len(name))
     cbc, bc, out = _pattern(name, rng)

@@ -230,8 +230,9 @@ __global__ void __launch_bounds__(256)
 
 bool fused_supported(int n) {
   switch (n / 2) {
-    case 16: case 24: case 32: case 40: case 48: case 64: case 80: case 96: case 128:
-    case 160: case 192: case 256: case 320: case 384: case 512: case 1024:
+    case 16: case 24: case 32: case 40: case 48: case 50: case 64: case 80: case 96:
+    case 100: case 128: case 160: case 192: case 200: case 256: case 320: case 384:
+    case 400: case 512: case 1024:
       return (n % 2) == 0;
     default:
       return false;
@@ -244,6 +245,10 @@ bool fused_supported(int n) {
     case 24: { constexpr int NC = 24; __VA_ARGS__; } break;   \
     case 32: { constexpr int NC = 32; __VA_ARGS__; } break;   \
     case 40: { constexpr int NC = 40; __VA_ARGS__; } break;   \
+    case 50: { constexpr int NC = 50; __VA_ARGS__; } break;   \
+    case 100: { constexpr int NC = 100; __VA_ARGS__; } break; \
+    case 200: { constexpr int NC = 200; __VA_ARGS__; } break; \
+    case 400: { constexpr int NC = 400; __VA_ARGS__; } break; \
     case 48: { constexpr int NC = 48; __VA_ARGS__; } break;   \
     case 64: { constexpr int NC = 64; __VA_ARGS__; } break;   \
     case 80: { constexpr int NC = 80; __VA_ARGS__; } break;   \
@@ -377,6 +382,10 @@ int fused_group_planes(int ny, int elem_bytes) {
       case 24: return group_planes<T, 24>();
       case 32: return group_planes<T, 32>();
       case 40: return group_planes<T, 40>();
+      case 50: return group_planes<T, 50>();
+      case 100: return group_planes<T, 100>();
+      case 200: return group_planes<T, 200>();
+      case 400: return group_planes<T, 400>();
       case 48: return group_planes<T, 48>();
       case 64: return group_planes<T, 64>();
       case 80: return group_planes<T, 80>();

@@ -189,6 +189,11 @@ FL_PLAN(40, 8, 5, 1, 2)
 FL_PLAN(80, 16, 5, 1, 2)
 FL_PLAN(160, 8, 4, 5, 3)
 FL_PLAN(320, 8, 8, 5, 3)
+// two radix-5 passes (real lines of 100, 200, 400 and 800 points)
+FL_PLAN(50, 2, 5, 5, 3)
+FL_PLAN(100, 4, 5, 5, 3)
+FL_PLAN(200, 8, 5, 5, 3)
+FL_PLAN(400, 16, 5, 5, 3)
 #undef FL_PLAN
 
 // XOR swizzle of the buffer between passes 0 and 1 (power-of-two r[0]):

@@ -108,6 +108,12 @@ SOLVE_CASES = [
     ((640, 32, 6), (1.0, 1.0, 1.0), "DDPPNN", 0.0),
     ((32, 640, 8), (1.0, 1.0, 2.0), "PPPPPP", 0.0),
     ((160, 320, 10), (1.0, 1.0, 1.0), "NNDDDD", 1.2),
+    # two radix-5 passes (100, 200, 400, 800 points)
+    ((200, 100, 12), (2 * np.pi, np.pi, 1.0), "PPPPNN", 1.5),
+    ((400, 32, 8), (1.0, 1.0, 1.0), "NNNNNN", 0.0),
+    ((32, 400, 8), (1.0, 1.0, 2.0), "PPPPPP", 0.0),
+    ((800, 32, 6), (1.0, 1.0, 1.0), "DDPPNN", 0.0),
+    ((100, 200, 10), (1.0, 1.0, 1.0), "NDDNNN", 1.2),
     # Neumann-Dirichlet pairs in x or y (REDFT11 / RODFT11, line pipeline)
     ((32, 24, 20), (1.0, 1.0, 1.0), "NDPPNN", 1.3),
     ((24, 32, 16), (1.0, 2.0, 1.0), "PPDNDD", 0.0),

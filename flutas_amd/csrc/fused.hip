@@ -366,7 +366,7 @@ int x_fwd_blocks(const LineTables &tx, int elem_bytes, int ny, int nz) {
   // tile_lines<T, NC>() evaluated at run time
   const int ls = tx.n + (elem_bytes == 8 ? 2 : 4);
   const int l = kTileBytes / (ls * elem_bytes);
-  const int tj = (tx.n == 1024 && elem_bytes == 8)
+  const int tj = ((tx.n == 1024 || tx.n == 400) && elem_bytes == 8)
                      ? 16
                      : (l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1))));
   return ((ny + tj - 1) / tj) * nz;

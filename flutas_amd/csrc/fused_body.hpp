@@ -65,11 +65,14 @@ __device__ __forceinline__ void st_nt2(T x, T y, T *p) {
 // twiddles, one workgroup per CU, whose transposed rows then cover whole
 // 128-byte lines (1024x1024x512 FP64 solve: 16 lines and 8-plane slab
 // groups 9.18 ms; 8 lines / 4 planes, also one workgroup per CU, 10.40 ms;
-// 4 lines / 2 planes, two per CU, 10.82 ms).
+// 4 lines / 2 planes, two per CU, 10.82 ms).  So do FP64 lines of 400
+// points (400^3 solve 1.37 -> 1.29 ms, P1 0.24 -> 0.20 ms); for 640- and
+// 768-point lines 16-line tiles measured slower (P1 0.53 -> 0.62 ms,
+// 0.58 -> 0.71 ms).
 constexpr int kTileBytes = 72 * 1024;
 template <typename T, int NC>
 constexpr int tile_lines() {
-  if (NC == 512 && sizeof(T) == 8) return 16;
+  if ((NC == 512 || NC == 200) && sizeof(T) == 8) return 16;
   constexpr int bytes = line_stride<T, 2 * NC>() * (int)sizeof(T);
   constexpr int l = kTileBytes / bytes;
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
@@ -166,6 +169,7 @@ template <typename T, int NC>
 constexpr int tile_threads() {
   if (2 * NC == 512) return 512;
   if (2 * NC == 1024) return 512;
+  if (NC == 200 && sizeof(T) == 8) return 512;  // 16-line tiles (tile_lines)
   return 256;
 }
 

@@ -453,6 +453,36 @@ int launch_zy_bwd(const LineTables &ty, int kind, T *Wt, WStride ws,
   return FLUTAS_OK;
 }
 
+// Workgroups of the chunked P2 (*r2) and P3 (*r3) slab kernels that one CU
+// holds at once, from the runtime's occupancy calculator (it counts the
+// kernels' registers, AGPRs included, and their LDS).
+template <typename T>
+static int slab_residency(int ny, int periodic, int *r2, int *r3) {
+  FL_DISPATCH(ny / 2, {
+    constexpr int GP = group_planes<T, NC>();
+    constexpr int NTP = slab_threads<T, NC, true>();
+    constexpr int NTN = slab_threads<T, NC, false>();
+    if (periodic) {
+      FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          r2, reinterpret_cast<const void *>(&k_yz_fwd_zc<T, NC, GP, NTP, true>), NTP, 0));
+      FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          r3, reinterpret_cast<const void *>(&k_zy_bwd_zc<T, NC, GP, NTP, true>), NTP, 0));
+    } else {
+      FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          r2, reinterpret_cast<const void *>(&k_yz_fwd_zc<T, NC, GP, NTN, false>), NTN, 0));
+      FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          r3, reinterpret_cast<const void *>(&k_zy_bwd_zc<T, NC, GP, NTN, false>), NTN, 0));
+    }
+  });
+  return FLUTAS_OK;
+}
+
+int fused_slab_residency(int ny, int elem_bytes, int periodic, int *r2, int *r3) {
+  *r2 = *r3 = 0;
+  return elem_bytes == 8 ? slab_residency<double>(ny, periodic, r2, r3)
+                         : slab_residency<float>(ny, periodic, r2, r3);
+}
+
 size_t zchunk_doubles(int nx, int ny, int ngrp, int nchunk, bool per) {
   const size_t col = (size_t)nx * ny;
   return col * (3 * (size_t)ngrp + (size_t)nchunk * zc_tab_slots(per) + 1 +

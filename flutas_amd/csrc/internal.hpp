@@ -148,6 +148,9 @@ __host__ __device__ inline WStride fused_layout_fp64(int nx, int ny, int nz) {
 }
 bool fused_supported(int n);
 int fused_group_planes(int ny, int elem_bytes);
+// Workgroups per CU of the chunked P2 (*r2) and P3 (*r3) slab kernels for
+// y-lines of ny points in storage of elem_bytes (occupancy calculator).
+int fused_slab_residency(int ny, int elem_bytes, int periodic, int *r2, int *r3);
 // Tables and side arrays of the z-chunked slab passes (fused_chunk.hpp):
 // the planes of an x-slab are cut into nchunk chunks of cpl planes (a
 // multiple of the plane group; the last chunk runs to nz).  Index forms:

@@ -350,16 +350,50 @@ int set_zchunks(flutas_psolve *h, int n) {
   return FLUTAS_OK;
 }
 
-// Default chunking: 512 slab workgroups or more (two per CU, as many as
-// the LDS of the 256- and 512-point passes holds at once) when the plan
-// has fewer x-slabs, with chunks of 32 planes or more.  Measured FP64 at
-// 256x256x512: 1 chunk 0.78 ms, 2 chunks 0.56 ms, 4 chunks 0.58 ms; at
-// 128^3: 0.115, 0.082, 0.065 (4), 0.072 (8) ms; at 512^3 whole slabs are
-// fastest.
+// Default chunking: the count that fills the GPU's workgroup slots best
+// over the two slab passes.  A pass of W = nx * n workgroups, R of which
+// the GPU holds at once (CUs x residency per CU from the occupancy
+// calculator), runs ceil(W / R) rounds and keeps W / (ceil(W / R) R) of
+// the slots busy; the score is the mean of that fill over P2 and P3, ties
+// go to the smaller count, and every chunk keeps 32 planes or more.
+// Measured FP64 solves (ms for 1/2/3/4/6/8 chunks; * the count taken):
+//   768x768x256 NN-z      2.81  2.69* 2.72  2.71  2.79  2.79
+//   640x640x320 NN-z      2.85  2.55  2.56  2.49* 2.59  2.57
+//   640x640x256 periodic  2.49  2.13  2.14  2.10* 2.17  2.18
+//   384x768x256 DD-z      1.54  1.38  1.42  1.32* 1.38  1.38
+//   512^3 NN-z            1.92* 1.90  1.92  1.98  1.96  1.95
+//   1024x1024x512 NN-z    9.68* 10.0  10.1  10.1  10.3  10.5
+//   256x256x512 periodic  0.78  0.55* 0.63  0.61  0.62  0.60
+//   400^3 NN-z            1.26  1.23  1.15  1.19  1.16  1.21  (5 taken)
+// and at 128^3: 0.115, 0.082 (2), 0.065 (4, taken), 0.072 (8).
 int default_zchunks(const flutas_psolve *h) {
-  int n = 1;
-  while (n < kMaxZChunks && h->nx * n < 512 && h->nz / (2 * n) >= 32) n *= 2;
-  return n;
+  if (!h->fused) return 1;
+  const int elem = (h->precision == FLUTAS_FP64) ? 8 : 4;
+  const int per = (h->cbc[4] == 'P');
+  int r2 = 0, r3 = 0, dev = 0, ncu = 0;
+  if (fused_slab_residency(h->ny, elem, per, &r2, &r3) != FLUTAS_OK || r2 < 1 || r3 < 1 ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      ncu < 1) {
+    (void)hipGetLastError();
+    return 1;
+  }
+  auto fill = [&](long long w, long long r) {
+    const long long rounds = (w + r - 1) / r;
+    return (double)w / (double)(rounds * r);
+  };
+  int best = 1;
+  double best_score = -1.0;
+  for (int n = 1; n <= kMaxZChunks; ++n) {
+    if (n > 1 && h->nz / n < 32) break;
+    const long long w = (long long)h->nx * n;
+    const double score = 0.5 * (fill(w, (long long)ncu * r2) + fill(w, (long long)ncu * r3));
+    if (score > best_score + 1e-9) {
+      best_score = score;
+      best = n;
+    }
+  }
+  return best;
 }
 
 // One spectral solve in storage/arithmetic T: the right-hand side `in`

@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
             const T *__restrict__ twh, const T *__restrict__ twq,
             double *__restrict__ absmax_part, int *nf_flag, int nf_seq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
-  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  __shared__ __align__(16) T stw[x_global_tw<T, NC>() ? 2 : Tw<T, NC>::total];
   const XTile xt = x_tile_of((ny + TJ - 1) / TJ);
   double *slot = absmax_part ? absmax_part + blockIdx.x : nullptr;
   x_fwd_tile<T, Tin, NC, TJ, NT>(in, is1, is2, Wt, ws, ny, kind, tw, twh, twq, slot, sm, stw,
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(NT, slab_waves_per_simd(NT))
             long long os2, int ny, int kind, T scale, const T *__restrict__ tw,
             const T *__restrict__ twh, const T *__restrict__ twq) {
   __shared__ __align__(16) T sm[TJ * line_stride<T, 2 * NC>()];
-  __shared__ __align__(16) T stw[Tw<T, NC>::total];
+  __shared__ __align__(16) T stw[x_global_tw<T, NC>() ? 2 : Tw<T, NC>::total];
   const XTile xt = x_tile_of((ny + TJ - 1) / TJ);
   x_bwd_tile<T, Tout, NC, TJ, NT, ACC>(Wt, ws, out, os1, os2, ny, kind, scale, tw, twh, twq, sm,
                                        stw, xt.j0t * TJ, xt.k);
@@ -366,9 +366,9 @@ int x_fwd_blocks(const LineTables &tx, int elem_bytes, int ny, int nz) {
   // tile_lines<T, NC>() evaluated at run time
   const int ls = tx.n + (elem_bytes == 8 ? 2 : 4);
   const int l = kTileBytes / (ls * elem_bytes);
-  const int tj = ((tx.n == 1024 || tx.n == 400) && elem_bytes == 8)
-                     ? 16
-                     : (l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1))));
+  int tj = l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
+  if (elem_bytes == 8 && tx.n == 1024) tj = 8;
+  if (elem_bytes == 8 && tx.n == 400) tj = 16;
   return ((ny + tj - 1) / tj) * nz;
 }
 

@@ -61,22 +61,31 @@ __device__ __forceinline__ void st_nt2(T x, T y, T *p) {
 
 // Lines per tile: the largest of 16, 8, 4, 2, 1 whose LDS tile fits
 // kTileBytes (two tiles per CU fit the 160 KiB of gfx950 LDS).  FP64
-// lines of 1024 points take 16 lines anyway: 131 KB plus 25.6 KB of staged
-// twiddles, one workgroup per CU, whose transposed rows then cover whole
-// 128-byte lines (1024x1024x512 FP64 solve: 16 lines and 8-plane slab
-// groups 9.18 ms; 8 lines / 4 planes, also one workgroup per CU, 10.40 ms;
-// 4 lines / 2 planes, two per CU, 10.82 ms).  So do FP64 lines of 400
-// points (400^3 solve 1.37 -> 1.29 ms, P1 0.24 -> 0.20 ms); for 640- and
+// lines of 1024 points take 8 lines (65.7 KB) and read their twiddles from
+// global memory (x_global_tw) instead of staging 25.6 KB of them, so two
+// tiles share a CU and each transposed row still covers a whole 64-byte
+// half line (1024x1024x512 FP64: P1 2.42 -> 2.13 ms, P4 2.48 -> 2.36 ms,
+// solve 9.58 -> 9.12 ms, against 16 staged-twiddle lines at one tile per
+// CU; 16 lines had measured 9.18 ms against 8 staged-twiddle lines at
+// 10.40 ms, also one per CU).  FP64 lines of 400 points take 16 lines
+// (400^3 solve 1.37 -> 1.29 ms, P1 0.24 -> 0.20 ms); for 640- and
 // 768-point lines 16-line tiles measured slower (P1 0.53 -> 0.62 ms,
 // 0.58 -> 0.71 ms).
 constexpr int kTileBytes = 72 * 1024;
 template <typename T, int NC>
 constexpr int tile_lines() {
-  if ((NC == 512 || NC == 200) && sizeof(T) == 8) return 16;
+  if (NC == 200 && sizeof(T) == 8) return 16;
+  if (NC == 512 && sizeof(T) == 8) return 8;
   constexpr int bytes = line_stride<T, 2 * NC>() * (int)sizeof(T);
   constexpr int l = kTileBytes / bytes;
   return l >= 16 ? 16 : (l >= 8 ? 8 : (l >= 4 ? 4 : (l >= 2 ? 2 : 1)));
 }
+
+// The P1/P4 tiles of 1024-point FP64 lines read the twiddle tables from
+// global memory (a tile has no prefetch for them to drain), which leaves
+// LDS for two tiles per CU.
+template <typename T, int NC>
+constexpr bool x_global_tw() { return NC == 512 && sizeof(T) == 8; }
 
 // Twiddle tables staged in LDS once per workgroup: exp(-2 pi i m/N) (N),
 // exp(-2 pi i k/n) (N+1) and exp(-i pi k/2n) (N+1, the entries the real
@@ -154,10 +163,11 @@ __device__ __forceinline__ void block_max_store(double am, double *dst) {
 
 // Minimum waves per SIMD of a workgroup size: two 512-thread workgroups
 // per CU (4 waves per SIMD, at most 128 VGPRs each).  For 512-point lines
-// the LDS of two tiles or slabs (with their staged twiddles) fits a CU;
-// a 1024-point FP64 tile or slab group (16 lines or 2 x 8 planes of 1026
-// doubles, 131 KB, plus 25.6 KB of staged twiddles) fills one CU's LDS, so
-// those run one workgroup (two waves per SIMD) per CU.
+// the LDS of two tiles or slabs (with their staged twiddles) fits a CU,
+// and so do two 1024-point FP64 tiles (8 lines, twiddles from global
+// memory); a 1024-point FP64 slab group (2 x 8 planes of 1026 doubles,
+// 131 KB, plus 25.6 KB of staged twiddles) fills one CU's LDS, so those
+// run one workgroup (two waves per SIMD) per CU.
 constexpr int slab_waves_per_simd(int nt) { return nt >= 512 ? 4 : 1; }
 
 // Threads of a P1/P4 tile workgroup: 512 for lines of 512 and 1024 points
@@ -221,8 +231,8 @@ __device__ __forceinline__ void x_fwd_tile(const Tin *__restrict__ in, long long
     }
     sm[l * LS + pos] = x;
   }
-  const Tw<T, NC> W =
-      stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw);
+  const Tw<T, NC> W = x_global_tw<T, NC>() ? Tw<T, NC>{tw, twh, twq}
+      : (stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw));
   if (absmax_slot) {
     // max |in| over the tile (the clamped loads only repeat tile entries),
     // NaN propagating
@@ -316,8 +326,8 @@ __device__ __forceinline__ void x_bwd_tile(const T *__restrict__ Wt, WStride ws,
       sm[(t % TJ) * LS + pos] = x;
     }
   }
-  const Tw<T, NC> W =
-      stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw);
+  const Tw<T, NC> W = x_global_tw<T, NC>() ? Tw<T, NC>{tw, twh, twq}
+      : (stage_tw ? stage_twiddles<T, NC, NT>(stw, tw, twh, twq) : twiddle_view<T, NC>(stw));
   __syncthreads();
   if (d4) {
     tile::d4_pre<T, NC, TJ, NT, LS, true>(sm, W.th);

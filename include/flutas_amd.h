@@ -148,8 +148,9 @@ int flutas_psolve_time_stage(flutas_psolve *h, const double *f, double *p, int s
  * n chunks (1..8) by separate workgroups and joined by a reduced system of
  * two unknowns per chunk and column, which gives plans with few x-slabs
  * more workgroups than the GPU has CUs.  n = 0 restores the plan's default
- * (doubling from 1 while nx * n < 512 and the chunks keep 32 planes or
- * more); n = 1 walks whole slabs.  The count may come out lower than n for
+ * (the count, with chunks of 32 planes or more, that best fills the GPU's
+ * workgroup slots over the two slab passes, from the CU count and the
+ * kernels' occupancy); n = 1 walks whole slabs.  The count may come out lower than n for
  * a short z.  Results equal the whole-slab walk's to rounding; the pivots
  * stay the reference's bit for bit.  Changing the count frees the tables a
  * recorded graph of this plan's solve refers to: record again after it.

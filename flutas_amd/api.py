@@ -251,13 +251,13 @@ class PoissonPlan:
         return n.value
 
     def tune(self, f, p, reps=5):
-        """Times `reps` solves f -> p at 1, 2, 4, 8 z chunks, keeps the
-        fastest (flutas_psolve_tune); returns (chosen, {count: ms})."""
+        """Times `reps` solves f -> p at 1..8 z chunks, keeps the fastest
+        (flutas_psolve_tune); returns (chosen, {count: ms})."""
         chosen = C.c_int()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * 8)()
         _check(lib.flutas_psolve_tune(_VP(self._h), f.c, p.c, C.c_int(reps),
                                       C.byref(chosen), ms))
-        return chosen.value, {n: ms[v] for v, n in enumerate((1, 2, 4, 8)) if ms[v] >= 0}
+        return chosen.value, {v + 1: ms[v] for v in range(8) if ms[v] >= 0}
 
     def rhsb_device(self):
         rx, ry, rz = _D(), _D(), _D()

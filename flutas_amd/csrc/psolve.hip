@@ -1091,15 +1091,15 @@ int flutas_psolve_tune(flutas_psolve *h, const double *f, double *p, int reps, i
   if (!h || !f || !p || !chosen || reps < 1 || reps > 100)
     return fail(FLUTAS_EINVAL, "bad tuning arguments");
   hipStream_t s = current_stream();
-  const int counts[4] = {1, 2, 4, 8};
   int best = 1;
   double tbest = 1e300;
   // the first timed count should not pay for the GPU's clock ramp
   for (int r = 0; r < 3; ++r) FL_TRY(solve_impl(h, f, p));
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < kMaxZChunks; ++v) {
+    const int n = v + 1;
     if (ms) ms[v] = -1.0;
-    FL_TRY(set_zchunks(h, counts[v]));
-    if (h->zchunks != counts[v]) continue;
+    FL_TRY(set_zchunks(h, n));
+    if (h->zchunks != n) continue;
     FL_TRY(solve_impl(h, f, p));  // warm-up
     FL_HIP(hipStreamSynchronize(s));
     const auto t0 = std::chrono::steady_clock::now();
@@ -1110,7 +1110,7 @@ int flutas_psolve_tune(flutas_psolve *h, const double *f, double *p, int reps, i
     if (ms) ms[v] = t;
     if (t < tbest) {
       tbest = t;
-      best = counts[v];
+      best = n;
     }
   }
   FL_TRY(set_zchunks(h, best));

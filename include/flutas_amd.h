@@ -159,10 +159,10 @@ int flutas_psolve_set_zchunks(flutas_psolve *h, int n);
 int flutas_psolve_zchunks(const flutas_psolve *h, int *n);
 
 /* Autotuner of the single solve (no counterpart in the reference): times
- * `reps` (1..100) solves f -> p at each chunk count 1, 2, 4, 8 that the
- * grid allows, keeps the fastest and returns it in *chosen; ms (may be
- * null) receives the mean milliseconds per solve for 1, 2, 4, 8 (-1 where
- * the count was not possible).  p holds a solution afterwards. */
+ * `reps` (1..100) solves f -> p at each chunk count 1..8 that the grid
+ * allows, keeps the fastest and returns it in *chosen; ms (may be null)
+ * receives 8 values, the mean milliseconds per solve for 1..8 chunks (-1
+ * where the count was not possible).  p holds a solution afterwards. */
 int flutas_psolve_tune(flutas_psolve *h, const double *f, double *p, int reps, int *chosen,
                        double *ms);
 

@@ -11,7 +11,8 @@ on the same seeded right-hand sides:
     71-77), periodic z (cyclic closure), the 512-point blocked work field;
   * FP32 + FP64 refinement with chunks: relative residual <= 1e-10 and
     <= 1e-9 against the oracle;
-  * the tuner picks one of the timed counts and leaves a correct solution.
+  * the tuner times every count 1..8 the grid allows, picks one of them
+    and leaves a correct solution.
 """
 import numpy as np
 import pytest
@@ -115,6 +116,9 @@ def test_tuner_keeps_a_timed_count(api, oracle):
     print("tune", flush=True)
     chosen, ms = plan.tune(f, p, reps=3)
     assert chosen in ms and 1 in ms and all(v > 0 for v in ms.values())
+    # every count the grid allows is timed, not only powers of two (96
+    # planes in groups of 16: 3 chunks of 32 planes)
+    assert 3 in ms and set(ms) <= set(range(1, 9))
     assert plan.zchunks == chosen
     assert relmax(interior(p.to_host()), interior(pref)) <= 1e-12
     plan.solve_into(f, p)

@@ -355,7 +355,13 @@ int set_zchunks(flutas_psolve *h, int n) {
 // the GPU holds at once (CUs x residency per CU from the occupancy
 // calculator), runs ceil(W / R) rounds and keeps W / (ceil(W / R) R) of
 // the slots busy; the score is the mean of that fill over P2 and P3, ties
-// go to the smaller count, and every chunk keeps 32 planes or more.
+// go to the smaller count, and every chunk keeps 32 planes or more.  The
+// FP32 chain of the refined mode pays 0.05 of score per chunk beyond the
+// first: its chunk tables and side arrays stay FP64 against half the field
+// bytes, so a chunk costs it relatively more (refined 256x256x512
+// periodic: 1/2/3/4/6/8 chunks 1.71, 1.30, 1.36, 1.31, 1.37, 1.34 ms; the
+// unpenalised score took 6, the penalised one takes 2; refined 128^3:
+// 0.309, 0.244, 0.230, 0.207, 0.249, 0.225 ms, 4 either way).
 // Measured FP64 solves (ms for 1/2/3/4/6/8 chunks; * the count taken):
 //   768x768x256 NN-z      2.81  2.69* 2.72  2.71  2.79  2.79
 //   640x640x320 NN-z      2.85  2.55  2.56  2.49* 2.59  2.57
@@ -387,7 +393,8 @@ int default_zchunks(const flutas_psolve *h) {
   for (int n = 1; n <= kMaxZChunks; ++n) {
     if (n > 1 && h->nz / n < 32) break;
     const long long w = (long long)h->nx * n;
-    const double score = 0.5 * (fill(w, (long long)ncu * r2) + fill(w, (long long)ncu * r3));
+    const double score = 0.5 * (fill(w, (long long)ncu * r2) + fill(w, (long long)ncu * r3)) -
+                         (elem == 4 ? 0.05 * (n - 1) : 0.0);
     if (score > best_score + 1e-9) {
       best_score = score;
       best = n;

@@ -289,6 +289,7 @@ def main():
 
     # ---- FP64 ---------------------------------------------------------
     plan = api.PoissonPlan(case.ng, case.dli, dzci, dzfi, case.cbc)
+    z_chunks = plan.zchunks
     t64 = time_steps(lambda: plan.solve_into(f, p), args.steps, args.warmup, api)
     ms = t64["mean"]
     relres64 = plan.residual(f, p)
@@ -363,7 +364,8 @@ def main():
                                f"tanh z stretching {case.alpha}, seeded N(0,1) RHS "
                                "(dz-weighted mean removed), inputs resident in HBM",
                    "ng": list(case.ng), "bc": case.cbc,
-                   "pipeline": "five-pass line path" if args.line_path else "four fused passes"},
+                   "pipeline": "five-pass line path" if args.line_path else "four fused passes",
+                   "z_chunks": z_chunks},
         "roofline": {"bound": "hbm", "kernel": dom["stage"],
                      "time_source": ("HIP events between the stages of whole solves "
                                      "(flutas_psolve_profile)" if args.line_path else
